@@ -1,0 +1,98 @@
+"""Writes the golden fixtures under tests/golden/ from the reference's own
+test data.  Nothing of the reference is imported or executed:
+
+* ``tests/unittest_data/fqe_data/*.npy`` are raw float64 dumps
+  (``ndarray.tofile``, reference
+  ``tests/unittest_data/fqe_data/generate_data.py:47-95``; read back with
+  ``numpy.fromfile`` as ``tests/unittest_data/fqe_data_loader.py:28-31``
+  does).  Copied as arrays: the FqeData(2,3,6), (2,1,4) and (1,1,2)
+  coefficients, h1, h2 and the FQE outputs of apply((h1,)), apply((0,h2))
+  and apply((h1,h2)).
+* literal tables are read from the reference test sources with
+  ``ast.literal_eval`` on their literal nodes: strings and addresses of
+  ``tests/fci_graph_test.py:33-112``, the n=4 maps of ``:151-190``, and the
+  eigenvector / eigenvalue of ``tests/fqe_data_test.py:761-826``.
+
+Run from the repository root:  python tests/golden/make_fixtures.py
+"""
+
+import ast
+import json
+import os
+
+import numpy
+
+REF = "/root/reference/tests"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+SECTORS = [(2, 3, 6), (2, 1, 4), (1, 1, 2)]
+
+
+def raw(name):
+    return numpy.fromfile(os.path.join(REF, "unittest_data", "fqe_data", name))
+
+
+def fqe_data_arrays():
+    from math import comb
+    arrays = {}
+    for na, nb, n in SECTORS:
+        tag = f"{na:02d}{nb:02d}{n:02d}"
+        shape = (comb(n, na), comb(n, nb))
+        arrays[f"c_{tag}"] = (raw(f"cr{tag}.npy") + 1j * raw(f"ci{tag}.npy")
+                              ).reshape(shape)
+        arrays[f"h1_{tag}"] = raw(f"h1{tag}.npy").reshape(n, n)
+        arrays[f"h2_{tag}"] = raw(f"h2{tag}.npy").reshape(n, n, n, n)
+        for suffix in ("1", "2", "12"):
+            arrays[f"out{suffix}_{tag}"] = (
+                raw(f"cr{tag}_{suffix}.npy") + 1j * raw(f"ci{tag}_{suffix}.npy")
+            ).reshape(shape)
+    return arrays
+
+
+def _find_assign(tree, func_name, var):
+    for node in ast.walk(tree):
+        if isinstance(node, ast.FunctionDef) and node.name == func_name:
+            for sub in ast.walk(node):
+                if (isinstance(sub, ast.Assign) and len(sub.targets) == 1
+                        and isinstance(sub.targets[0], ast.Name)
+                        and sub.targets[0].id == var):
+                    return sub.value
+    raise KeyError((func_name, var))
+
+
+def _literal(node):
+    # numpy.array([...], dtype=...) / numpy.asarray(...): take the first arg
+    if isinstance(node, ast.Call):
+        node = node.args[0]
+    return ast.literal_eval(node)
+
+
+def literal_tables():
+    with open(os.path.join(REF, "fci_graph_test.py")) as f:
+        tree = ast.parse(f.read())
+    refdata = _literal(_find_assign(tree, "test_fci_graph", "refdata"))
+    refdict = _literal(_find_assign(tree, "test_fci_graph", "refdict"))
+    amap = _literal(_find_assign(tree, "test_fci_graph_maps", "ref_alpha_map"))
+    bmap = _literal(_find_assign(tree, "test_fci_graph_maps", "ref_beta_map"))
+    with open(os.path.join(REF, "fqe_data_test.py")) as f:
+        tree = ast.parse(f.read())
+    wfn = _literal(_find_assign(tree, "test_2_body", "wfn"))
+    scale = _literal(_find_assign(tree, "test_2_body", "scale"))
+    return {
+        "strings_n8_k4": refdata,
+        "index_n8_k4": {str(k): v for k, v in refdict.items()},
+        "maps_n4_k2": {f"{i},{j}": v for (i, j), v in amap.items()},
+        "maps_n4_k1": {f"{i},{j}": v for (i, j), v in bmap.items()},
+        "eigvec_2_2_4": [[[z.real, z.imag] for z in row] for row in wfn],
+        "eigval_2_2_4": scale,
+    }
+
+
+def main():
+    numpy.savez(os.path.join(OUT, "fqe_data_golden.npz"), **fqe_data_arrays())
+    with open(os.path.join(OUT, "literal_tables.json"), "w") as f:
+        json.dump(literal_tables(), f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
