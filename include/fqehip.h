@@ -1,0 +1,207 @@
+/*
+ * fqehip — MI355X (gfx950) engine for the determinant-space action of a
+ * spin-restricted 1+2-body Hamiltonian on a CI vector, i.e. the hot path of
+ * OpenFermion-FQE's Wavefunction.apply(RestrictedHamiltonian) and
+ * Wavefunction.time_evolve.
+ *
+ * C ABI.  Conventions for every entry point:
+ *   - extern "C"; plain pointers and int64 sizes, no torch types;
+ *   - d_* arguments are device pointers owned by the caller;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *     work is enqueued, nothing synchronises unless stated;
+ *   - return 0 (FQEHIP_OK) or an FQEHIP_ERR_* code; fqehip_last_error()
+ *     gives the message of the calling thread's last failure.  Nothing
+ *     exits the process (the reference's safe_malloc does,
+ *     src/fqe/lib/macros.c:64-75).
+ *
+ * Data layouts (device memory):
+ *   strings   uint64[L], address order (lexicographic rank of the ascending
+ *             occupation tuple), as FciGraph._astr;
+ *   dexc      int32[L][nexc][3] = (source, i*norb+j, sign) rows per target,
+ *             ascending i*norb+j, nexc = nele*(norb-nele+1), as
+ *             FciGraph._dexca;
+ *   maps      int32[sum_ij m_ij][3], map (i,j) at offsets[i*norb+j],
+ *             ascending source, as FciGraph._alpha_map[(i,j)];
+ *   pair table int32[npair][L]: entry for (pair p, target I) is
+ *             sign*(source+1) where <I|E_p|source> = sign, or 0.  Ordered
+ *             basis: npair = norb^2, p = i*norb+j, E_p = E_ij.  Compressed
+ *             basis: npair = norb(norb+1)/2, p = j + i(i+1)/2 (i >= j),
+ *             E_p = E_ij + E_ji (i != j) or E_ii;
+ *   coefficients  complex, interleaved (re, im), row-major [lena][lenb]:
+ *             complex128 for FQEHIP_C128, complex64 for FQEHIP_C64;
+ *   dvec / Y  planar real, [plane][re|im][lena*lenb], double for
+ *             FQEHIP_C128, float for FQEHIP_C64.
+ */
+#ifndef FQEHIP_H_
+#define FQEHIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FQEHIP_OK 0
+#define FQEHIP_ERR_INVALID 1     /* bad size, shape or pointer */
+#define FQEHIP_ERR_OOM 2         /* a device allocation failed */
+#define FQEHIP_ERR_UNSUPPORTED 3 /* e.g. norb above the supported maximum */
+#define FQEHIP_ERR_HIP 4         /* a HIP runtime or launch error */
+
+/* Storage / arithmetic modes. */
+#define FQEHIP_C128 0 /* complex FP64 storage, FP64 MFMA GEMM */
+#define FQEHIP_C64 1  /* complex FP32 storage, BF16 hi/lo split MFMA GEMM
+                         with FP32 accumulation; FP64 norms / Taylor sum */
+
+const char* fqehip_last_error(void);
+int fqehip_version(void);
+
+/* ---------------------------------------------------------------------
+ * (a) String tables, built on the GPU.
+ * Replaces FciGraph._build_strings (src/fqe/fci_graph.py:301-333 ->
+ * src/fqe/lib/bitstring.c:29-46, src/fqe/lib/fci_graph.c:27-54,136-147)
+ * and map_to_deexc (src/fqe/fci_graph.py:36-63 -> fci_graph.c:57-73).
+ * d_strings: uint64[nstrings]; d_dexc: int32[nstrings][nele*(norb-nele+1)][3].
+ * ------------------------------------------------------------------- */
+int fqehip_string_tables(int norb, int nele, int64_t nstrings,
+                         uint64_t* d_strings, int32_t* d_dexc, void* stream);
+
+/* Excitation maps a+_i a_j for all (i,j).  Replaces
+ * FciGraph._build_mapping (src/fqe/fci_graph.py:204-244 ->
+ * src/fqe/lib/fci_graph.py:60-99 -> fci_graph.c:76-121).
+ * d_offsets: int64[norb*norb+1] start row of each map (host-computed
+ * prefix sums of C(norb-2,nele-1) / C(norb-1,nele-1)); d_maps: int32[*][3]. */
+int fqehip_string_maps(int norb, int nele, int64_t nstrings,
+                       const uint64_t* d_strings, const int64_t* d_offsets,
+                       int32_t* d_maps, void* stream);
+
+/* Dense pair table (layout above) from dexc; compressed = 0 or 1. */
+int fqehip_pair_table(int norb, int nele, int64_t nstrings,
+                      const int32_t* d_dexc, int compressed, int32_t* d_ptab,
+                      void* stream);
+
+/* ---------------------------------------------------------------------
+ * (b) dvec: D[p] = E_p C for all pairs p (pair basis of the tables).
+ * Replaces FqeData.calculate_dvec_spatial (src/fqe/fqe_data.py:2149-2160,
+ * :2209-2234 -> zdvec_make, src/fqe/lib/fqe_data.c:350-377) and, for the
+ * compressed basis, _calculate_dvec_spatial_compressed (:2336-2353).
+ * d_dvec: planar [npair][2][lena*lenb].
+ * ------------------------------------------------------------------- */
+int fqehip_dvec_gather(int precision, int npair, int64_t lena, int64_t lenb,
+                       const int32_t* d_ptab_a, const int32_t* d_ptab_b,
+                       const void* d_coeff, void* d_dvec, void* stream);
+
+/* ---------------------------------------------------------------------
+ * (c) Y = A B on the matrix cores: A real row-major m x k (FP64),
+ * B row-major k x ncol and Y row-major m x ncol in the storage precision.
+ * Replaces the dvec contraction einsum('ijkl,klmn->ijmn') of
+ * src/fqe/fqe_data.py:656 (and :671-672 for the compressed basis); the
+ * one-body term is one more row of A.
+ * ------------------------------------------------------------------- */
+int fqehip_pair_gemm(int precision, int m, int k, const double* d_a,
+                     int64_t ncol, const void* d_b, void* d_y, void* stream);
+
+/* ---------------------------------------------------------------------
+ * (d) Scatter + epilogue:
+ *   v[I] = alpha * (Y[npair][I] + sum_p (E_p Y[p])[I]) + beta * cin[I]
+ *   out[I] = v[I];  acc[I] += v[I] (complex128, optional);
+ *   norm_partials[block] = partial sums of |v|^2 in FP64 (optional).
+ * Y has npair+1 planes (the last holds the one-body term).  Replaces
+ * _calculate_coeff_spatial_with_dvec (src/fqe/fqe_data.py:2309-2334 ->
+ * zcoeff_make, src/fqe/lib/fqe_data.c:379-406) plus the ax_plus_y of
+ * src/fqe/wavefunction.py:396-397 and :564.  If d_done is not NULL and
+ * *d_done != 0 the kernel does nothing (device-side early exit of the
+ * Taylor loop).  d_cin may alias d_out only when beta == 0.
+ * ------------------------------------------------------------------- */
+int fqehip_scatter(int precision, int npair, int64_t lena, int64_t lenb,
+                   const int32_t* d_ptab_a, const int32_t* d_ptab_b,
+                   const void* d_y, double alpha_re, double alpha_im,
+                   double beta_re, double beta_im, const void* d_cin,
+                   void* d_out, double* d_acc, double* d_norm_partials,
+                   const int32_t* d_done, void* stream);
+
+/* Number of partial sums fqehip_scatter / fqehip_apply1 write for a
+ * sector of n = lena*lenb determinants. */
+int64_t fqehip_norm_partials_count(int64_t n);
+
+/* One-body apply out = alpha * sum_ij h1[i,j] E_ij C + beta * cin.
+ * Replaces _apply_array_spatial1 (src/fqe/fqe_data.py:477-531 ->
+ * lm_apply_array1, src/fqe/lib/fqe_data.c:272-302).  d_h1: complex128
+ * [npair] in the pair basis of the tables. */
+int fqehip_apply1(int precision, int npair, int64_t lena, int64_t lenb,
+                  const int32_t* d_ptab_a, const int32_t* d_ptab_b,
+                  const double* d_h1, const void* d_coeff, double alpha_re,
+                  double alpha_im, double beta_re, double beta_im,
+                  const void* d_cin, void* d_out, double* d_acc,
+                  double* d_norm_partials, const int32_t* d_done,
+                  void* stream);
+
+/* Reduce norm partials: norm = sqrt(sum partials) (FP64, fixed order).
+ * d_norm_out[0] = norm.  If d_done is not NULL (Taylor mode) and
+ * *d_done == 0: when norm < accuracy, sets *d_done = 1 and
+ * d_nterms[0] = term; otherwise leaves them. */
+int fqehip_norm_finalize(const double* d_partials, int64_t npartials,
+                         double* d_norm_out, double accuracy, int term,
+                         int32_t* d_done, int32_t* d_nterms, void* stream);
+
+/* Elementwise helpers for the Taylor loop and conversions. */
+/* out(complex, storage precision) = in(complex128) */
+int fqehip_convert_from_c128(int precision, int64_t n, const double* d_in,
+                             void* d_out, void* stream);
+/* out(complex128) = scale * in(complex, storage precision) */
+int fqehip_convert_to_c128(int precision, int64_t n, const void* d_in,
+                           double scale_re, double scale_im, double* d_out,
+                           void* stream);
+
+/* ---------------------------------------------------------------------
+ * Composed apply and Taylor propagation on one sector.
+ * ------------------------------------------------------------------- */
+typedef struct fqehip_sector {
+  int norb, nalpha, nbeta;
+  int npair;               /* pair-basis size of the tables below */
+  int64_t lena, lenb;
+  const int32_t* ptab_a;   /* [npair][lena] */
+  const int32_t* ptab_b;   /* [npair][lenb] */
+} fqehip_sector;
+
+typedef struct fqehip_operator {
+  int complex_op;   /* 0: A is (npair+1) x npair; 1: 2(npair+1) x 2npair */
+  const double* a;  /* device, row-major */
+  const double* h1; /* one-body only apply: complex128 [npair], else NULL */
+} fqehip_operator;
+
+/* Bytes of dvec and Y workspace for one apply of a sector. */
+int64_t fqehip_apply_workspace_bytes(int precision, const fqehip_sector* s);
+
+/* out = alpha * H C + beta * cin (see fqehip_scatter for acc/partials).
+ * Replaces FqeData.apply_inplace for (h1, h2) tuples
+ * (src/fqe/fqe_data.py:421-475, :582-710) and Wavefunction._apply_array
+ * (src/fqe/wavefunction.py:366-399).  d_workspace holds
+ * fqehip_apply_workspace_bytes(); d_coeff may alias d_out only if
+ * beta == 0. */
+int fqehip_apply(int precision, const fqehip_sector* s,
+                 const fqehip_operator* op, const void* d_coeff,
+                 double alpha_re, double alpha_im, double beta_re,
+                 double beta_im, const void* d_cin, void* d_out,
+                 double* d_acc, double* d_norm_partials,
+                 const int32_t* d_done, void* d_workspace, void* stream);
+
+/* Taylor series psi(t) = sum_k (-i t H)^k psi / k! for a real operator
+ * (time folded into the scalar), reference
+ * src/fqe/wavefunction.py:556-568.  d_term (storage precision) holds psi
+ * on entry and is overwritten; d_result (complex128) receives the sum.
+ * Stops when ||term_k|| < accuracy or after expansion-1 terms; the
+ * number of terms and the converged flag are written to d_status[0..1]
+ * (int32: nterms, converged) without a host synchronisation.  Scratch:
+ * d_workspace (fqehip_apply_workspace_bytes), d_partials
+ * (fqehip_norm_partials_count doubles), d_norm (1 double). */
+int fqehip_taylor(int precision, const fqehip_sector* s,
+                  const fqehip_operator* op, double time, double accuracy,
+                  int expansion, void* d_term, double* d_result,
+                  int32_t* d_status, double* d_partials, double* d_norm,
+                  void* d_workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FQEHIP_H_ */

@@ -1,0 +1,662 @@
+// (b) dvec gather, (c) pair GEMM on the matrix cores, (d) gather-form
+// scatter with the Taylor epilogue, the one-body apply, norm reduction and
+// the composed apply / Taylor entry points.
+//
+// The apply follows the dvec form of the operator
+//   H C = sum_p h1'[p] D[p] + sum_p E_p Y[p],   D[p] = E_p C,   Y = V D
+// (reference src/fqe/fqe_data.py:646-683).  Both D = E C and the scatter
+// sum_p E_p Y[p] are gathers at the target determinant through the same
+// pair table (entry (p, I) = sign * (source + 1) with <I|E_p|source> =
+// sign), so no kernel needs atomics.  The one-body term is the last row of
+// the GEMM's A matrix and lands in plane npair of Y.
+#include "fqehip_common.h"
+
+#include <cmath>
+
+namespace fqehip {
+namespace {
+
+constexpr int kBlock = 256;
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// idx -> (row, column) of a lena x lenb matrix without a 64-bit integer
+// division: a double estimate corrected by one step.
+__device__ __forceinline__ void split_index(int64_t idx, int64_t lenb,
+                                            double inv_lenb, int64_t& ia,
+                                            int64_t& ib) {
+  ia = static_cast<int64_t>(static_cast<double>(idx) * inv_lenb);
+  ib = idx - ia * lenb;
+  if (ib < 0) {
+    --ia;
+    ib += lenb;
+  } else if (ib >= lenb) {
+    ++ia;
+    ib -= lenb;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T signed_val(int32_t code, T v) {
+  return code > 0 ? v : -v;
+}
+
+// ---------------------------------------------------------------------------
+// (b) dvec gather: D[p][re|im][I] for one plane p per blockIdx.y.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_dvec_gather(int64_t lena, int64_t lenb, double inv_lenb,
+              const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+              const T* __restrict__ coeff, T* __restrict__ dvec,
+              const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t n = lena * lenb;
+  const int p = blockIdx.y;
+  const int32_t* pa = pta + static_cast<int64_t>(p) * lena;
+  const int32_t* pb = ptb + static_cast<int64_t>(p) * lenb;
+  T* dre = dvec + static_cast<int64_t>(2 * p) * n;
+  T* dim = dre + n;
+  for (int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       idx < n; idx += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int64_t ia, ib;
+    split_index(idx, lenb, inv_lenb, ia, ib);
+    T re = 0, im = 0;
+    const int32_t a = pa[ia];
+    if (a) {
+      const int64_t src = (a > 0 ? a : -a) - 1;
+      const T* c = coeff + 2 * (src * lenb + ib);
+      re += signed_val(a, c[0]);
+      im += signed_val(a, c[1]);
+    }
+    const int32_t b = pb[ib];
+    if (b) {
+      const int64_t src = (b > 0 ? b : -b) - 1;
+      const T* c = coeff + 2 * (ia * lenb + src);
+      re += signed_val(b, c[0]);
+      im += signed_val(b, c[1]);
+    }
+    dre[idx] = re;
+    dim[idx] = im;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (c) FP64 GEMM Y = A B on v_mfma_f64_16x16x4f64.
+// A: m x k (row-major, read through the cache; it is tiny and shared by all
+// workgroups).  B: k x ncol, Y: m x ncol, row-major.  A wave owns one 16-
+// column tile and RT 16-row tiles; a workgroup is 4 waves = 64 columns.
+// MFMA operand layout (16x16x4 f64, 64 lanes): A[r = lane%16][k = lane/16],
+// B[k = lane/16][c = lane%16]; result item i of lane l is
+// Y[l/16 + 4*i][l%16] (rows interleaved, unlike the f32-output MFMAs).
+// ---------------------------------------------------------------------------
+constexpr int kGemmRowTiles = 8;
+
+__global__ void __launch_bounds__(kBlock)
+k_gemm_f64(int m, int k, const double* __restrict__ a, int64_t ncol,
+           const double* __restrict__ b, double* __restrict__ y,
+           const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t col0 = (static_cast<int64_t>(blockIdx.x) * 4 + wave) * 16;
+  if (col0 >= ncol) return;
+  const int row0 = blockIdx.y * kGemmRowTiles * 16;
+  const int64_t col = col0 + (lane & 15);
+  const bool colok = col < ncol;
+  const int kl = lane >> 4;
+  double4_t acc[kGemmRowTiles];
+#pragma unroll
+  for (int t = 0; t < kGemmRowTiles; ++t) acc[t] = double4_t{0, 0, 0, 0};
+  int ntiles = (m - row0 + 15) / 16;
+  if (ntiles > kGemmRowTiles) ntiles = kGemmRowTiles;
+  for (int k0 = 0; k0 < k; k0 += 4) {
+    const int kk = k0 + kl;
+    const bool kok = kk < k;
+    const double bv = (colok && kok) ? b[static_cast<int64_t>(kk) * ncol + col] : 0.0;
+#pragma unroll
+    for (int t = 0; t < kGemmRowTiles; ++t) {
+      if (t < ntiles) {
+        const int r = row0 + t * 16 + (lane & 15);
+        const double av = (r < m && kok) ? a[static_cast<int64_t>(r) * k + kk] : 0.0;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  if (!colok) return;
+#pragma unroll
+  for (int t = 0; t < kGemmRowTiles; ++t) {
+    if (t < ntiles) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = row0 + t * 16 + (lane >> 4) + 4 * i;
+        if (r < m) y[static_cast<int64_t>(r) * ncol + col] = acc[t][i];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// (d) scatter + epilogue (see fqehip_scatter).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void epilogue(int64_t idx, double re, double im,
+                                         double alpha_re, double alpha_im,
+                                         double beta_re, double beta_im,
+                                         const T* __restrict__ cin,
+                                         T* __restrict__ out,
+                                         double* __restrict__ acc,
+                                         double& sq) {
+  double vr = alpha_re * re - alpha_im * im;
+  double vi = alpha_re * im + alpha_im * re;
+  if (beta_re != 0.0 || beta_im != 0.0) {
+    const double cr = static_cast<double>(cin[2 * idx]);
+    const double ci = static_cast<double>(cin[2 * idx + 1]);
+    vr += beta_re * cr - beta_im * ci;
+    vi += beta_re * ci + beta_im * cr;
+  }
+  const T or_ = static_cast<T>(vr);
+  const T oi = static_cast<T>(vi);
+  out[2 * idx] = or_;
+  out[2 * idx + 1] = oi;
+  if (acc) {
+    acc[2 * idx] += vr;
+    acc[2 * idx + 1] += vi;
+  }
+  // the norm is taken of what was stored
+  const double sr = static_cast<double>(or_);
+  const double si = static_cast<double>(oi);
+  sq += sr * sr + si * si;
+}
+
+__device__ __forceinline__ void block_sum_store(double v, double* partials) {
+  __shared__ double red[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+    partials[blockIdx.x] = s;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_scatter(int npair, int64_t lena, int64_t lenb, double inv_lenb,
+          const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+          const T* __restrict__ y, double alpha_re, double alpha_im,
+          double beta_re, double beta_im, const T* cin, T* out,
+          double* __restrict__ acc, double* __restrict__ partials,
+          const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t n = lena * lenb;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (idx < n) {
+    int64_t ia, ib;
+    split_index(idx, lenb, inv_lenb, ia, ib);
+    const T* yh = y + static_cast<int64_t>(2 * npair) * n;
+    double re = static_cast<double>(yh[idx]);
+    double im = static_cast<double>(yh[n + idx]);
+    for (int p = 0; p < npair; ++p) {
+      const T* yre = y + static_cast<int64_t>(2 * p) * n;
+      const int32_t a = pta[static_cast<int64_t>(p) * lena + ia];
+      if (a) {
+        const int64_t off = ((a > 0 ? a : -a) - 1) * lenb + ib;
+        re += signed_val(a, static_cast<double>(yre[off]));
+        im += signed_val(a, static_cast<double>(yre[n + off]));
+      }
+      const int32_t b = ptb[static_cast<int64_t>(p) * lenb + ib];
+      if (b) {
+        const int64_t off = ia * lenb + (b > 0 ? b : -b) - 1;
+        re += signed_val(b, static_cast<double>(yre[off]));
+        im += signed_val(b, static_cast<double>(yre[n + off]));
+      }
+    }
+    epilogue(idx, re, im, alpha_re, alpha_im, beta_re, beta_im, cin, out, acc,
+             sq);
+  }
+  if (partials) block_sum_store(sq, partials);
+}
+
+// One-body apply: out = alpha * sum_p h1[p] (E_p C) + beta * cin.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_apply1(int npair, int64_t lena, int64_t lenb, double inv_lenb,
+         const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+         const double* __restrict__ h1, const T* __restrict__ coeff,
+         double alpha_re, double alpha_im, double beta_re, double beta_im,
+         const T* cin, T* out, double* __restrict__ acc,
+         double* __restrict__ partials, const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t n = lena * lenb;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (idx < n) {
+    int64_t ia, ib;
+    split_index(idx, lenb, inv_lenb, ia, ib);
+    double re = 0.0, im = 0.0;
+    for (int p = 0; p < npair; ++p) {
+      const double hr = h1[2 * p], hi = h1[2 * p + 1];
+      double sr = 0.0, si = 0.0;
+      const int32_t a = pta[static_cast<int64_t>(p) * lena + ia];
+      if (a) {
+        const int64_t off = 2 * (((a > 0 ? a : -a) - 1) * lenb + ib);
+        sr += signed_val(a, static_cast<double>(coeff[off]));
+        si += signed_val(a, static_cast<double>(coeff[off + 1]));
+      }
+      const int32_t b = ptb[static_cast<int64_t>(p) * lenb + ib];
+      if (b) {
+        const int64_t off = 2 * (ia * lenb + (b > 0 ? b : -b) - 1);
+        sr += signed_val(b, static_cast<double>(coeff[off]));
+        si += signed_val(b, static_cast<double>(coeff[off + 1]));
+      }
+      re += hr * sr - hi * si;
+      im += hr * si + hi * sr;
+    }
+    epilogue(idx, re, im, alpha_re, alpha_im, beta_re, beta_im, cin, out, acc,
+             sq);
+  }
+  if (partials) block_sum_store(sq, partials);
+}
+
+// Single-workgroup reduction in a fixed order; optional convergence test.
+__global__ void __launch_bounds__(kBlock)
+k_norm_finalize(const double* __restrict__ partials, int64_t np,
+                double* __restrict__ norm_out, double accuracy, int term,
+                int32_t* done, int32_t* nterms) {
+  __shared__ double red[kBlock];
+  if (done && *done) return;
+  double s = 0.0;
+  for (int64_t k = threadIdx.x; k < np; k += kBlock) s += partials[k];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double nrm = sqrt(red[0]);
+    norm_out[0] = nrm;
+    if (done) {
+      nterms[0] = term;
+      if (nrm < accuracy) *done = 1;
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_from_c128(int64_t n2, const double* __restrict__ in, T* __restrict__ out) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       k < n2; k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[k] = static_cast<T>(in[k]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_to_c128(int64_t n, const T* __restrict__ in, double sr, double si,
+          double* __restrict__ out) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+       k < n; k += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double a = static_cast<double>(in[2 * k]);
+    const double b = static_cast<double>(in[2 * k + 1]);
+    out[2 * k] = sr * a - si * b;
+    out[2 * k + 1] = sr * b + si * a;
+  }
+}
+
+__global__ void k_status_init(int32_t* status) {
+  if (threadIdx.x == 0) {
+    status[0] = 0;
+    status[1] = 0;
+  }
+}
+
+unsigned grid_stride_blocks(int64_t n) {
+  const unsigned b = blocks_for(n, kBlock);
+  return b < 65536u ? (b == 0 ? 1u : b) : 65536u;
+}
+
+int check_precision(int precision) {
+  FQEHIP_CHECK_ARG(precision == FQEHIP_C128 || precision == FQEHIP_C64,
+                   "unknown precision %d", precision);
+  return FQEHIP_OK;
+}
+
+template <typename T>
+int launch_gather(int npair, int64_t lena, int64_t lenb, const int32_t* pta,
+                  const int32_t* ptb, const void* coeff, void* dvec,
+                  hipStream_t st, const int32_t* done) {
+  const int64_t n = lena * lenb;
+  if (n == 0 || npair == 0) return FQEHIP_OK;
+  unsigned bx = blocks_for(n, kBlock);
+  if (bx > 8192u) bx = 8192u;
+  dim3 grid(bx, npair);
+  k_dvec_gather<T><<<grid, kBlock, 0, st>>>(lena, lenb, 1.0 / lenb, pta, ptb,
+                                           static_cast<const T*>(coeff),
+                                           static_cast<T*>(dvec), done);
+  FQEHIP_CHECK_LAUNCH("k_dvec_gather");
+  return FQEHIP_OK;
+}
+
+template <typename T>
+int launch_scatter(int npair, int64_t lena, int64_t lenb, const int32_t* pta,
+                   const int32_t* ptb, const void* y, double ar, double ai,
+                   double br, double bi, const void* cin, void* out,
+                   double* acc, double* partials, const int32_t* done,
+                   hipStream_t st) {
+  const int64_t n = lena * lenb;
+  if (n == 0) return FQEHIP_OK;
+  k_scatter<T><<<blocks_for(n, kBlock), kBlock, 0, st>>>(
+      npair, lena, lenb, 1.0 / lenb, pta, ptb, static_cast<const T*>(y), ar,
+      ai, br, bi, static_cast<const T*>(cin), static_cast<T*>(out), acc,
+      partials, done);
+  FQEHIP_CHECK_LAUNCH("k_scatter");
+  return FQEHIP_OK;
+}
+
+template <typename T>
+int launch_apply1(int npair, int64_t lena, int64_t lenb, const int32_t* pta,
+                  const int32_t* ptb, const double* h1, const void* coeff,
+                  double ar, double ai, double br, double bi, const void* cin,
+                  void* out, double* acc, double* partials,
+                  const int32_t* done, hipStream_t st) {
+  const int64_t n = lena * lenb;
+  if (n == 0) return FQEHIP_OK;
+  k_apply1<T><<<blocks_for(n, kBlock), kBlock, 0, st>>>(
+      npair, lena, lenb, 1.0 / lenb, pta, ptb, h1,
+      static_cast<const T*>(coeff), ar, ai, br, bi,
+      static_cast<const T*>(cin), static_cast<T*>(out), acc, partials, done);
+  FQEHIP_CHECK_LAUNCH("k_apply1");
+  return FQEHIP_OK;
+}
+
+int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
+                    const double* b, double* y, hipStream_t st,
+                    const int32_t* done) {
+  if (m == 0 || ncol == 0) return FQEHIP_OK;
+  dim3 grid(blocks_for(ncol, 64), (m + kGemmRowTiles * 16 - 1) / (kGemmRowTiles * 16));
+  k_gemm_f64<<<grid, kBlock, 0, st>>>(m, k, a, ncol, b, y, done);
+  FQEHIP_CHECK_LAUNCH("k_gemm_f64");
+  return FQEHIP_OK;
+}
+
+}  // namespace
+
+// Declared in gemm_bf16.hip.
+int launch_gemm_bf16x3(int m, int k, const double* a, int64_t ncol,
+                       const float* b, float* y, hipStream_t st,
+                       const int32_t* done);
+
+namespace {
+
+int launch_gemm(int precision, int m, int k, const double* a, int64_t ncol,
+                const void* b, void* y, hipStream_t st, const int32_t* done) {
+  if (precision == FQEHIP_C128)
+    return launch_gemm_f64(m, k, a, ncol, static_cast<const double*>(b),
+                           static_cast<double*>(y), st, done);
+  return launch_gemm_bf16x3(m, k, a, ncol, static_cast<const float*>(b),
+                            static_cast<float*>(y), st, done);
+}
+
+int launch_gather_any(int precision, int npair, int64_t lena, int64_t lenb,
+                      const int32_t* pta, const int32_t* ptb,
+                      const void* coeff, void* dvec, hipStream_t st,
+                      const int32_t* done) {
+  if (precision == FQEHIP_C128)
+    return launch_gather<double>(npair, lena, lenb, pta, ptb, coeff, dvec, st,
+                                 done);
+  return launch_gather<float>(npair, lena, lenb, pta, ptb, coeff, dvec, st,
+                              done);
+}
+
+}  // namespace
+
+}  // namespace fqehip
+
+using namespace fqehip;
+
+extern "C" int fqehip_dvec_gather(int precision, int npair, int64_t lena,
+                                  int64_t lenb, const int32_t* d_ptab_a,
+                                  const int32_t* d_ptab_b, const void* d_coeff,
+                                  void* d_dvec, void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(npair >= 0 && lena >= 0 && lenb >= 0, "negative size");
+  FQEHIP_CHECK_ARG(d_ptab_a && d_ptab_b && d_coeff && d_dvec, "NULL pointer");
+  return launch_gather_any(precision, npair, lena, lenb, d_ptab_a, d_ptab_b,
+                           d_coeff, d_dvec, as_stream(stream), nullptr);
+}
+
+extern "C" int fqehip_pair_gemm(int precision, int m, int k, const double* d_a,
+                                int64_t ncol, const void* d_b, void* d_y,
+                                void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(m >= 0 && k >= 0 && ncol >= 0, "negative size");
+  FQEHIP_CHECK_ARG(d_a && d_b && d_y, "NULL pointer");
+  return launch_gemm(precision, m, k, d_a, ncol, d_b, d_y, as_stream(stream),
+                     nullptr);
+}
+
+extern "C" int64_t fqehip_norm_partials_count(int64_t n) {
+  return n > 0 ? static_cast<int64_t>(blocks_for(n, kBlock)) : 0;
+}
+
+extern "C" int fqehip_scatter(int precision, int npair, int64_t lena,
+                              int64_t lenb, const int32_t* d_ptab_a,
+                              const int32_t* d_ptab_b, const void* d_y,
+                              double alpha_re, double alpha_im, double beta_re,
+                              double beta_im, const void* d_cin, void* d_out,
+                              double* d_acc, double* d_norm_partials,
+                              const int32_t* d_done, void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(npair >= 0 && lena >= 0 && lenb >= 0, "negative size");
+  FQEHIP_CHECK_ARG(d_ptab_a && d_ptab_b && d_y && d_out, "NULL pointer");
+  FQEHIP_CHECK_ARG((beta_re == 0.0 && beta_im == 0.0) || d_cin,
+                   "beta != 0 needs d_cin");
+  if (precision == FQEHIP_C128)
+    return launch_scatter<double>(npair, lena, lenb, d_ptab_a, d_ptab_b, d_y,
+                                  alpha_re, alpha_im, beta_re, beta_im, d_cin,
+                                  d_out, d_acc, d_norm_partials, d_done,
+                                  as_stream(stream));
+  return launch_scatter<float>(npair, lena, lenb, d_ptab_a, d_ptab_b, d_y,
+                               alpha_re, alpha_im, beta_re, beta_im, d_cin,
+                               d_out, d_acc, d_norm_partials, d_done,
+                               as_stream(stream));
+}
+
+extern "C" int fqehip_apply1(int precision, int npair, int64_t lena,
+                             int64_t lenb, const int32_t* d_ptab_a,
+                             const int32_t* d_ptab_b, const double* d_h1,
+                             const void* d_coeff, double alpha_re,
+                             double alpha_im, double beta_re, double beta_im,
+                             const void* d_cin, void* d_out, double* d_acc,
+                             double* d_norm_partials, const int32_t* d_done,
+                             void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(npair >= 0 && lena >= 0 && lenb >= 0, "negative size");
+  FQEHIP_CHECK_ARG(d_ptab_a && d_ptab_b && d_h1 && d_coeff && d_out,
+                   "NULL pointer");
+  FQEHIP_CHECK_ARG(d_coeff != d_out, "apply1 cannot run in place");
+  FQEHIP_CHECK_ARG((beta_re == 0.0 && beta_im == 0.0) || d_cin,
+                   "beta != 0 needs d_cin");
+  if (precision == FQEHIP_C128)
+    return launch_apply1<double>(npair, lena, lenb, d_ptab_a, d_ptab_b, d_h1,
+                                 d_coeff, alpha_re, alpha_im, beta_re, beta_im,
+                                 d_cin, d_out, d_acc, d_norm_partials, d_done,
+                                 as_stream(stream));
+  return launch_apply1<float>(npair, lena, lenb, d_ptab_a, d_ptab_b, d_h1,
+                              d_coeff, alpha_re, alpha_im, beta_re, beta_im,
+                              d_cin, d_out, d_acc, d_norm_partials, d_done,
+                              as_stream(stream));
+}
+
+extern "C" int fqehip_norm_finalize(const double* d_partials, int64_t npartials,
+                                    double* d_norm_out, double accuracy,
+                                    int term, int32_t* d_done,
+                                    int32_t* d_nterms, void* stream) {
+  clear_error();
+  FQEHIP_CHECK_ARG(d_norm_out && (npartials == 0 || d_partials), "NULL pointer");
+  FQEHIP_CHECK_ARG((d_done == nullptr) == (d_nterms == nullptr),
+                   "d_done and d_nterms go together");
+  k_norm_finalize<<<1, kBlock, 0, as_stream(stream)>>>(
+      d_partials, npartials, d_norm_out, accuracy, term, d_done, d_nterms);
+  FQEHIP_CHECK_LAUNCH("k_norm_finalize");
+  return FQEHIP_OK;
+}
+
+extern "C" int fqehip_convert_from_c128(int precision, int64_t n,
+                                        const double* d_in, void* d_out,
+                                        void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(n >= 0 && (n == 0 || (d_in && d_out)), "bad arguments");
+  if (n == 0) return FQEHIP_OK;
+  hipStream_t st = as_stream(stream);
+  if (precision == FQEHIP_C128)
+    k_from_c128<double><<<grid_stride_blocks(2 * n), kBlock, 0, st>>>(
+        2 * n, d_in, static_cast<double*>(d_out));
+  else
+    k_from_c128<float><<<grid_stride_blocks(2 * n), kBlock, 0, st>>>(
+        2 * n, d_in, static_cast<float*>(d_out));
+  FQEHIP_CHECK_LAUNCH("k_from_c128");
+  return FQEHIP_OK;
+}
+
+extern "C" int fqehip_convert_to_c128(int precision, int64_t n,
+                                      const void* d_in, double scale_re,
+                                      double scale_im, double* d_out,
+                                      void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(n >= 0 && (n == 0 || (d_in && d_out)), "bad arguments");
+  if (n == 0) return FQEHIP_OK;
+  hipStream_t st = as_stream(stream);
+  if (precision == FQEHIP_C128)
+    k_to_c128<double><<<grid_stride_blocks(n), kBlock, 0, st>>>(
+        n, static_cast<const double*>(d_in), scale_re, scale_im, d_out);
+  else
+    k_to_c128<float><<<grid_stride_blocks(n), kBlock, 0, st>>>(
+        n, static_cast<const float*>(d_in), scale_re, scale_im, d_out);
+  FQEHIP_CHECK_LAUNCH("k_to_c128");
+  return FQEHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Composed apply and Taylor loop.
+// ---------------------------------------------------------------------------
+namespace {
+
+int check_sector_op(int precision, const fqehip_sector* s,
+                    const fqehip_operator* op) {
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(s && op, "NULL sector or operator");
+  FQEHIP_CHECK_ARG(s->npair >= 0 && s->lena >= 0 && s->lenb >= 0,
+                   "negative sector size");
+  FQEHIP_CHECK_ARG(s->ptab_a && s->ptab_b, "sector without pair tables");
+  FQEHIP_CHECK_ARG(op->h1 || op->a, "operator has neither A nor h1");
+  FQEHIP_CHECK_ARG(op->complex_op == 0 || op->complex_op == 1,
+                   "complex_op=%d", op->complex_op);
+  return FQEHIP_OK;
+}
+
+size_t elem_size(int precision) {
+  return precision == FQEHIP_C128 ? sizeof(double) : sizeof(float);
+}
+
+}  // namespace
+
+extern "C" int64_t fqehip_apply_workspace_bytes(int precision,
+                                               const fqehip_sector* s) {
+  if (!s) return -1;
+  const int64_t n = s->lena * s->lenb;
+  return static_cast<int64_t>(2 * s->npair + 1) * 2 * n *
+         static_cast<int64_t>(elem_size(precision));
+}
+
+extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
+                            const fqehip_operator* op, const void* d_coeff,
+                            double alpha_re, double alpha_im, double beta_re,
+                            double beta_im, const void* d_cin, void* d_out,
+                            double* d_acc, double* d_norm_partials,
+                            const int32_t* d_done, void* d_workspace,
+                            void* stream) {
+  clear_error();
+  if (int rc = check_sector_op(precision, s, op)) return rc;
+  FQEHIP_CHECK_ARG(d_coeff && d_out, "NULL coefficient pointer");
+  if (op->h1)
+    return fqehip_apply1(precision, s->npair, s->lena, s->lenb, s->ptab_a,
+                         s->ptab_b, op->h1, d_coeff, alpha_re, alpha_im,
+                         beta_re, beta_im, d_cin, d_out, d_acc,
+                         d_norm_partials, d_done, stream);
+  FQEHIP_CHECK_ARG(d_workspace != nullptr, "NULL workspace");
+  const int64_t n = s->lena * s->lenb;
+  char* ws = static_cast<char*>(d_workspace);
+  void* dvec = ws;
+  void* y = ws + static_cast<int64_t>(s->npair) * 2 * n * elem_size(precision);
+  hipStream_t st = as_stream(stream);
+  int rc = launch_gather_any(precision, s->npair, s->lena, s->lenb, s->ptab_a,
+                             s->ptab_b, d_coeff, dvec, st, d_done);
+  if (rc) return rc;
+  if (op->complex_op)
+    rc = launch_gemm(precision, 2 * (s->npair + 1), 2 * s->npair, op->a, n,
+                     dvec, y, st, d_done);
+  else
+    rc = launch_gemm(precision, s->npair + 1, s->npair, op->a, 2 * n, dvec, y,
+                     st, d_done);
+  if (rc) return rc;
+  return fqehip_scatter(precision, s->npair, s->lena, s->lenb, s->ptab_a,
+                        s->ptab_b, y, alpha_re, alpha_im, beta_re, beta_im,
+                        d_cin, d_out, d_acc, d_norm_partials, d_done, stream);
+}
+
+extern "C" int fqehip_taylor(int precision, const fqehip_sector* s,
+                             const fqehip_operator* op, double time,
+                             double accuracy, int expansion, void* d_term,
+                             double* d_result, int32_t* d_status,
+                             double* d_partials, double* d_norm,
+                             void* d_workspace, void* stream) {
+  clear_error();
+  if (int rc = check_sector_op(precision, s, op)) return rc;
+  FQEHIP_CHECK_ARG(d_term && d_result && d_status && d_partials && d_norm,
+                   "NULL pointer");
+  FQEHIP_CHECK_ARG(d_workspace != nullptr, "NULL workspace");
+  hipStream_t st = as_stream(stream);
+  k_status_init<<<1, 64, 0, st>>>(d_status);
+  FQEHIP_CHECK_LAUNCH("k_status_init");
+  const int64_t n = s->lena * s->lenb;
+  const int64_t np = fqehip_norm_partials_count(n);
+  int32_t* nterms = d_status;
+  int32_t* done = d_status + 1;
+  // The one-body apply cannot run in place: its terms ping-pong between
+  // d_term and the workspace.
+  void* cur = d_term;
+  void* nxt = d_workspace;
+  for (int order = 1; order < expansion; ++order) {
+    // term <- (-i t / order) H term ;  result += term
+    const double ar = 0.0, ai = -time / static_cast<double>(order);
+    int rc;
+    if (op->h1) {
+      rc = fqehip_apply1(precision, s->npair, s->lena, s->lenb, s->ptab_a,
+                         s->ptab_b, op->h1, cur, ar, ai, 0.0, 0.0, nullptr,
+                         nxt, d_result, d_partials, done, stream);
+      void* t = cur;
+      cur = nxt;
+      nxt = t;
+    } else {
+      rc = fqehip_apply(precision, s, op, d_term, ar, ai, 0.0, 0.0, nullptr,
+                        d_term, d_result, d_partials, done, d_workspace,
+                        stream);
+    }
+    if (rc) return rc;
+    rc = fqehip_norm_finalize(d_partials, np, d_norm, accuracy, order, done,
+                              nterms, stream);
+    if (rc) return rc;
+  }
+  return FQEHIP_OK;
+}

@@ -1,0 +1,129 @@
+"""ctypes binding of libfqehip.so (the HIP kernels behind the C ABI of
+``include/fqehip.h``).
+
+torch is imported first so that the HIP runtime torch ships
+(``libamdhip64.so.7``) is the one the library binds to: device pointers and
+streams from torch are then valid in the library.  There is no CPU
+fallback: a missing library or a missing GPU raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_int32, c_int64, c_void_p
+
+import torch  # noqa: F401  (must precede the CDLL load, see module doc)
+
+LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                       "libfqehip.so")
+
+C128 = 0
+C64 = 1
+
+OK = 0
+ERR_INVALID = 1
+ERR_OOM = 2
+ERR_UNSUPPORTED = 3
+ERR_HIP = 4
+
+
+class FqeHipError(RuntimeError):
+    """A fqehip C-ABI call returned a non-zero status."""
+
+
+class Sector(ctypes.Structure):
+    _fields_ = [("norb", c_int), ("nalpha", c_int), ("nbeta", c_int),
+                ("npair", c_int), ("lena", c_int64), ("lenb", c_int64),
+                ("ptab_a", c_void_p), ("ptab_b", c_void_p)]
+
+
+class Operator(ctypes.Structure):
+    _fields_ = [("complex_op", c_int), ("a", c_void_p), ("h1", c_void_p)]
+
+
+_SIGNATURES = {
+    "fqehip_last_error": (ctypes.c_char_p, []),
+    "fqehip_version": (c_int, []),
+    "fqehip_string_tables": (c_int, [c_int, c_int, c_int64, c_void_p,
+                                     c_void_p, c_void_p]),
+    "fqehip_string_maps": (c_int, [c_int, c_int, c_int64, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]),
+    "fqehip_pair_table": (c_int, [c_int, c_int, c_int64, c_void_p, c_int,
+                                  c_void_p, c_void_p]),
+    "fqehip_dvec_gather": (c_int, [c_int, c_int, c_int64, c_int64, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fqehip_pair_gemm": (c_int, [c_int, c_int, c_int, c_void_p, c_int64,
+                                 c_void_p, c_void_p, c_void_p]),
+    "fqehip_scatter": (c_int, [c_int, c_int, c_int64, c_int64, c_void_p,
+                               c_void_p, c_void_p, c_double, c_double,
+                               c_double, c_double, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fqehip_norm_partials_count": (c_int64, [c_int64]),
+    "fqehip_apply1": (c_int, [c_int, c_int, c_int64, c_int64, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_double,
+                              c_double, c_double, c_double, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p]),
+    "fqehip_norm_finalize": (c_int, [c_void_p, c_int64, c_void_p, c_double,
+                                     c_int, c_void_p, c_void_p, c_void_p]),
+    "fqehip_convert_from_c128": (c_int, [c_int, c_int64, c_void_p, c_void_p,
+                                         c_void_p]),
+    "fqehip_convert_to_c128": (c_int, [c_int, c_int64, c_void_p, c_double,
+                                       c_double, c_void_p, c_void_p]),
+    "fqehip_apply_workspace_bytes": (c_int64, [c_int, POINTER(Sector)]),
+    "fqehip_apply": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
+                             c_void_p, c_double, c_double, c_double, c_double,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_void_p]),
+    "fqehip_taylor": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
+                              c_double, c_double, c_int, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it is not built or no GPU is present."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise FqeHipError(
+                f"{LIBPATH} is missing: build it (make -C openfermion-fqe_amd)")
+        handle = ctypes.CDLL(LIBPATH)
+        for name, (restype, argtypes) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = handle
+    return _lib
+
+
+def device() -> torch.device:
+    """The GPU the engine runs on; raises without one (no CPU fallback)."""
+    if not torch.cuda.is_available():
+        raise FqeHipError("fqe_amd needs an AMD GPU (torch.cuda is not "
+                          "available); there is no CPU fallback")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def ptr(t) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+def call(name: str, *args) -> int:
+    rc = getattr(lib(), name)(*args)
+    if name.endswith("_count") or name.endswith("_bytes"):
+        return rc
+    if rc != OK:
+        msg = lib().fqehip_last_error().decode(errors="replace")
+        if rc == ERR_INVALID:
+            raise ValueError(f"{name}: {msg}")
+        raise FqeHipError(f"{name} failed with status {rc}: {msg}")
+    return rc

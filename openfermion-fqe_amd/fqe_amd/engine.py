@@ -1,0 +1,166 @@
+"""Device form of the operator and the per-sector apply / Taylor drivers.
+
+The operator of a spatial (h1, h2) tuple is put in the dvec form the
+reference's Python path uses (``src/fqe/fqe_data.py:646-683``):
+
+    H C = sum_ij h1'[i,j] E_ij C + sum_ijkl V[ij,kl] E_ij E_kl C,
+    V[i,j,k,l] = -h2[i,k,j,l],  h1' = h1 - einsum('ikkj', V).
+
+Pair basis.  When V[ij,kl] = V[ji,kl] = V[ij,lk] and h1' is symmetric
+(real molecular integrals) the n(n+1)/2 unordered pairs carry the same
+operator exactly (reference ``:659-681`` assumes this for any real h; here
+it is checked, so a real non-symmetric h takes the ordered n^2 basis and
+matches the reference's C path).  GEMM matrix A: rows = output pairs plus
+one row for h1', columns = input pairs; complex operators use the real
+2x2-block form [[Re, -Im], [Im, Re]] on the planar (re, im) dvec.
+"""
+
+from __future__ import annotations
+
+from typing import Optional, Sequence, Tuple
+
+import numpy
+import torch
+
+from . import _lib
+
+
+class PreparedOperator:
+    """GEMM matrix (2-body) or pair vector (1-body) on the device."""
+
+    def __init__(self, norb: int, compressed: bool, complex_op: bool,
+                 a: Optional[torch.Tensor], h1: Optional[torch.Tensor]):
+        self.norb = norb
+        self.compressed = compressed
+        self.complex_op = complex_op
+        self.a = a
+        self.h1 = h1
+        self.npair = norb * (norb + 1) // 2 if compressed else norb * norb
+
+    def c_struct(self) -> _lib.Operator:
+        return _lib.Operator(int(self.complex_op), _lib.ptr(self.a),
+                             _lib.ptr(self.h1))
+
+
+def _pair_indices(norb: int, compressed: bool):
+    """(i, j) of each pair-basis index (representative i >= j when
+    compressed, index j + i(i+1)/2)."""
+    if compressed:
+        return [(i, j) for i in range(norb) for j in range(i + 1)]
+    return [(i, j) for i in range(norb) for j in range(norb)]
+
+
+def _is_pair_symmetric(h1p: numpy.ndarray, v: numpy.ndarray) -> bool:
+    return (numpy.array_equal(v, v.transpose(1, 0, 2, 3))
+            and numpy.array_equal(v, v.transpose(0, 1, 3, 2))
+            and numpy.array_equal(h1p, h1p.T))
+
+
+def prepare_operator(tensors: Sequence[numpy.ndarray], norb: int,
+                     device: torch.device) -> PreparedOperator:
+    """Device form of a 1- or 2-tensor spatial operator."""
+    if len(tensors) == 1:
+        h1 = numpy.asarray(tensors[0], dtype=numpy.complex128)
+        if h1.shape != (norb, norb):
+            raise ValueError("Inconsistent number of spin-orbitals in "
+                             "operators and wavefunction.")
+        compressed = numpy.array_equal(h1, h1.T)
+        pairs = _pair_indices(norb, compressed)
+        vec = numpy.array([h1[i, j] for i, j in pairs], dtype=numpy.complex128)
+        d = torch.from_numpy(vec.view(numpy.float64).copy()).to(device)
+        return PreparedOperator(norb, compressed, bool(numpy.any(vec.imag)),
+                                None, d)
+    if len(tensors) != 2:
+        raise ValueError("the GPU engine applies 1- and 2-body restricted "
+                         "operators")
+    h1e, h2e = tensors
+    if h1e is None:
+        h1e = numpy.zeros((norb, norb))
+    h1e = numpy.asarray(h1e)
+    h2e = numpy.asarray(h2e)
+    assert h1e.shape == (norb, norb)
+    assert h2e.shape == (norb, norb, norb, norb)
+    v = -numpy.moveaxis(h2e.astype(numpy.complex128), 1, 2)
+    h1p = h1e.astype(numpy.complex128) - numpy.einsum("ikkj->ij", v)
+    compressed = _is_pair_symmetric(h1p, v)
+    complex_op = bool(numpy.any(v.imag) or numpy.any(h1p.imag))
+    pairs = _pair_indices(norb, compressed)
+    rows = [i * norb + j for i, j in pairs]
+    vm = v.reshape(norb * norb, norb * norb)[numpy.ix_(rows, rows)]
+    h1row = h1p.reshape(-1)[rows]
+    full = numpy.vstack([vm, h1row[None, :]])  # (npair+1) x npair, complex
+    if complex_op:
+        m, k = full.shape
+        a = numpy.zeros((2 * m, 2 * k))
+        a[0::2, 0::2] = full.real
+        a[0::2, 1::2] = -full.imag
+        a[1::2, 0::2] = full.imag
+        a[1::2, 1::2] = full.real
+    else:
+        a = numpy.ascontiguousarray(full.real)
+    dev_a = torch.from_numpy(numpy.ascontiguousarray(a, dtype=numpy.float64)
+                             ).to(device)
+    return PreparedOperator(norb, compressed, complex_op, dev_a, None)
+
+
+def precision_of(dtype: torch.dtype) -> int:
+    if dtype == torch.complex128:
+        return _lib.C128
+    if dtype == torch.complex64:
+        return _lib.C64
+    raise TypeError(f"unsupported coefficient dtype {dtype}")
+
+
+def sector_struct(graph, op: PreparedOperator) -> Tuple[_lib.Sector, tuple]:
+    """C descriptor of a sector for the pair basis of ``op``; the second
+    element keeps the tables referenced."""
+    ta = graph.tables(True).pair_table(op.compressed)
+    tb = graph.tables(False).pair_table(op.compressed)
+    s = _lib.Sector(graph.norb(), graph.nalpha(), graph.nbeta(), op.npair,
+                    graph.lena(), graph.lenb(), _lib.ptr(ta), _lib.ptr(tb))
+    return s, (ta, tb)
+
+
+def workspace(sector: _lib.Sector, precision: int,
+              device: torch.device) -> torch.Tensor:
+    """Device scratch for one apply (dvec + Y planes); the one-body Taylor
+    loop uses it as its second term buffer."""
+    nbytes = int(_lib.lib().fqehip_apply_workspace_bytes(precision,
+                                                          _as_ref(sector)))
+    elem = 8 if precision == _lib.C128 else 4
+    need = (2 * sector.npair + 1) * 2 * sector.lena * sector.lenb * elem
+    if nbytes != need:
+        raise _lib.FqeHipError(f"workspace size {nbytes} != {need}")
+    nbytes = max(nbytes, 16 * sector.lena * sector.lenb, 16)
+    return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def _as_ref(struct):
+    import ctypes
+    return ctypes.byref(struct)
+
+
+def apply_sector(graph, coeff: torch.Tensor, op: PreparedOperator,
+                 out: torch.Tensor, alpha: complex = 1.0, beta: complex = 0.0,
+                 cin: Optional[torch.Tensor] = None,
+                 acc: Optional[torch.Tensor] = None,
+                 partials: Optional[torch.Tensor] = None,
+                 done: Optional[torch.Tensor] = None,
+                 ws: Optional[torch.Tensor] = None) -> None:
+    """out = alpha * H coeff + beta * cin on one sector (device tensors)."""
+    prec = precision_of(coeff.dtype)
+    s, keep = sector_struct(graph, op)
+    if ws is None and op.h1 is None:
+        ws = workspace(s, prec, coeff.device)
+    opc = op.c_struct()
+    _lib.call("fqehip_apply", prec, _as_ref(s), _as_ref(opc),
+              _lib.ptr(coeff), float(numpy.real(alpha)),
+              float(numpy.imag(alpha)), float(numpy.real(beta)),
+              float(numpy.imag(beta)), _lib.ptr(cin), _lib.ptr(out),
+              _lib.ptr(acc), _lib.ptr(partials), _lib.ptr(done),
+              _lib.ptr(ws), _lib.stream_ptr())
+    del keep
+
+
+def norm_partials_count(n: int) -> int:
+    return int(_lib.lib().fqehip_norm_partials_count(n))

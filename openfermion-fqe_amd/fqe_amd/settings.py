@@ -1,0 +1,15 @@
+"""Engine settings.
+
+``default_dtype``: coefficient dtype of new sectors.  complex128 runs the
+FP64 engine; complex64 runs the mixed-precision engine (complex FP32
+storage, BF16 hi/lo split matrix-core GEMM with FP32 accumulation, FP64
+norms and Taylor accumulation).
+"""
+
+import numpy
+
+default_dtype = numpy.complex128
+
+# Maximum number of orbitals of the GPU string tables (uint64 strings,
+# int32-addressed tables).
+max_norb = 32

@@ -1,0 +1,354 @@
+"""Wavefunction: the sectors of a number- and Sz-conserving CI vector.
+
+Mirrors ``fqe.wavefunction.Wavefunction`` for the apply path (reference
+``src/fqe/wavefunction.py:49-118``, ``:241-399``, ``:509-611``,
+``:613-812``, ``:961-1054``, ``:1100-1133``): same constructor, ``apply``,
+``time_evolve``, ``apply_generated_unitary`` (Taylor), ``norm``,
+``normalize``, ``scale``, ``ax_plus_y``, ``set_wfn``, ``get_coeff``,
+``expectationValue`` and error behaviour (``TypeError`` for number-
+conservation mismatches, ``ValueError`` for sizes, ``RuntimeError`` when
+the Taylor series does not converge).
+"""
+
+from __future__ import annotations
+
+import copy
+import math
+from typing import Dict, KeysView, List, Optional, Tuple, Union
+
+import numpy
+import torch
+
+from . import _lib
+from . import engine
+from . import settings
+from .fqe_data import FqeData
+from .hamiltonians import Hamiltonian, RestrictedHamiltonian
+from .util import alpha_beta_electrons, validate_tuple, vdot
+
+
+def build_hamiltonian(ops, norb: int = 0, conserve_number: bool = True,
+                      e_0: complex = 0. + 0.j) -> Hamiltonian:
+    """Tuple -> RestrictedHamiltonian (reference
+    ``src/fqe/fqe_decorators.py:45-72``)."""
+    if isinstance(ops, Hamiltonian):
+        return ops
+    if isinstance(ops, tuple):
+        validate_tuple(ops)
+        if norb != 0 and ops[0].shape[0] == norb:
+            out = RestrictedHamiltonian(ops, e_0=e_0)
+            out._conserve_number = conserve_number
+            return out
+        raise NotImplementedError(
+            "spin-orbital (General) Hamiltonians are outside the GPU "
+            "engine's path")
+    raise TypeError('Expected FermionOperator'
+                    ' but received {}.'.format(type(ops)))
+
+
+class Wavefunction:
+    """reference ``src/fqe/wavefunction.py:49``"""
+
+    def __init__(self, param: Optional[List[List[int]]] = None,
+                 broken: Optional[Union[List[str], str]] = None,
+                 dtype=None) -> None:
+        self._symmetry_map: Dict[Tuple[int, int], Tuple[int, int]] = {}
+        self._conserved: Dict[str, int] = {}
+        self._conserve_spin = broken is None or 'spin' not in broken
+        self._conserve_number = broken is None or 'number' not in broken
+        self._norb = 0
+        self._civec: Dict[Tuple[int, int], FqeData] = {}
+        self._dtype = numpy.dtype(dtype if dtype is not None
+                                  else settings.default_dtype)
+        if not self._conserve_spin and not self._conserve_number:
+            raise TypeError('Number and spin non-conserving waveunfction is'
+                            ' the complete Fock space.')
+        if not self._conserve_number:
+            raise NotImplementedError(
+                "number-broken wavefunctions are outside the GPU engine's "
+                "path")
+        if param:
+            norbs = set(x[2] for x in param)
+            if len(norbs) != 1:
+                raise ValueError('Number of orbitals is not consistent')
+            self._norb = list(norbs)[0]
+            for p in param:
+                nalpha, nbeta = alpha_beta_electrons(p[0], p[1])
+                self._civec[(p[0], p[1])] = FqeData(nalpha, nbeta, self._norb,
+                                                    dtype=self._dtype)
+            self._conserved['n'] = param[0][0]
+            if self._conserve_spin:
+                self._conserved['s_z'] = param[0][1]
+
+    # -- containers ------------------------------------------------------------
+    def sector(self, key: Tuple[int, int]) -> FqeData:
+        return self._civec[key]
+
+    def sectors(self) -> KeysView[Tuple[int, int]]:
+        return self._civec.keys()
+
+    def conserve_number(self) -> bool:
+        return self._conserve_number
+
+    def conserve_spin(self) -> bool:
+        return self._conserve_spin
+
+    def norb(self) -> int:
+        return self._norb
+
+    def get_coeff(self, key: Tuple[int, int]) -> numpy.ndarray:
+        return self._civec[key].coeff
+
+    def set_wfn(self, strategy: str = 'ones',
+                raw_data: Optional[Dict[Tuple[int, int],
+                                        numpy.ndarray]] = None) -> None:
+        """reference ``src/fqe/wavefunction.py:779-811``"""
+        if strategy == 'from_data' and not raw_data:
+            raise ValueError('No data provided for set_wfn')
+        if strategy == 'from_data':
+            for key, data in raw_data.items():
+                self._civec[key].set_wfn(strategy='from_data', raw_data=data)
+        elif strategy == 'hartree-fock':
+            if len(self.sectors()) != 1:
+                raise ValueError(("Hartree-Fock wf initialization only works "
+                                  "with single sector wavefunctions"))
+            for sec in self._civec.values():
+                sec.set_wfn(strategy=strategy)
+        else:
+            for sec in self._civec.values():
+                sec.set_wfn(strategy=strategy)
+        if strategy == 'random':
+            self.normalize()
+
+    def __deepcopy__(self, memodict={}) -> 'Wavefunction':
+        new = Wavefunction.__new__(Wavefunction)
+        new.__dict__.update({k: copy.copy(v) for k, v in self.__dict__.items()
+                             if k != '_civec'})
+        new._civec = {k: copy.deepcopy(v) for k, v in self._civec.items()}
+        return new
+
+    # -- vector algebra --------------------------------------------------------
+    def norm(self) -> float:
+        total = 0.0
+        for sec in self._civec.values():
+            total += sec.norm() ** 2
+        return math.sqrt(total)
+
+    def normalize(self) -> None:
+        self.scale(1.0 / self.norm())
+
+    def scale(self, sval: complex) -> None:
+        for sec in self._civec.values():
+            sec.scale(sval)
+
+    def ax_plus_y(self, sval: complex, wfn: 'Wavefunction') -> None:
+        if self._civec.keys() != wfn._civec.keys():
+            raise ValueError('inconsistent sectors in Wavefunction.ax_plus_y')
+        for key in self._civec:
+            self._civec[key].ax_plus_y(sval, wfn._civec[key])
+
+    # -- the hot path ------------------------------------------------------------
+    def _check_number(self, hamil: Hamiltonian) -> None:
+        if not self._conserve_number or not hamil.conserve_number():
+            if self._conserve_number:
+                raise TypeError('Number non-conserving hamiltonian passed to'
+                                ' number conserving wavefunction')
+            if hamil.conserve_number():
+                raise TypeError('Number conserving hamiltonian passed to'
+                                ' number non-conserving wavefunction')
+
+    def apply(self, hamil) -> 'Wavefunction':
+        """H|psi> for a RestrictedHamiltonian (reference
+        ``src/fqe/wavefunction.py:314-364``, decorator
+        ``src/fqe/fqe_decorators.py:353-370``)."""
+        hamil = build_hamiltonian(hamil, norb=self.norb(),
+                                  conserve_number=self.conserve_number())
+        self._check_number(hamil)
+        if not isinstance(hamil, RestrictedHamiltonian):
+            raise NotImplementedError(
+                "the GPU engine applies RestrictedHamiltonian operators")
+        if hamil.dim() != self._norb:
+            raise ValueError('Hamiltonian has incorrect size:'
+                             ' expected {}'.format(self._norb) +
+                             ' provided {}'.format(hamil.dim()))
+        return self._apply_array(hamil.tensors(), hamil.e_0(), hamil)
+
+    def _apply_array(self, array, e_0: complex,
+                     hamil: Optional[RestrictedHamiltonian] = None
+                     ) -> 'Wavefunction':
+        """out = H psi + e_0 psi (reference ``src/fqe/wavefunction.py:
+        366-399``); the e_0 term is fused into the scatter epilogue."""
+        op = _prepared(hamil, array, self._norb)
+        out = copy.deepcopy(self)
+        beta = e_0 if numpy.abs(e_0) > 1.e-15 else 0.0
+        for sec in out._civec.values():
+            sec.apply_inplace_scaled(array, alpha=1.0, beta=beta, op=op)
+        return out
+
+    def expectationValue(self, ops, brawfn: 'Wavefunction' = None) -> complex:
+        """<bra|H|self> (reference ``src/fqe/wavefunction.py:1100-1133``)."""
+        if not isinstance(ops, Hamiltonian):
+            raise TypeError('Expected an Fqe Hamiltonian or Operator'
+                            ' but recieved {}'.format(type(ops)))
+        work = self.apply(ops)
+        return vdot(brawfn if brawfn else self, work)
+
+    def time_evolve(self, time: float, hamil,
+                    inplace: bool = False) -> 'Wavefunction':
+        """exp(-i t H) |psi> (reference ``src/fqe/wavefunction.py:961-1054``).
+
+        Non-quadratic restricted Hamiltonians take the Taylor branch, as in
+        the reference.  Quadratic ones (h1 only), which the reference
+        evolves by an orbital rotation, are evolved here with the same
+        Taylor series of the one-body apply.  The e_0 phase is applied
+        twice, as the reference does (``:604-605`` and ``:1051-1052``)."""
+        hamil = build_hamiltonian(hamil, norb=self.norb(),
+                                  conserve_number=self.conserve_number())
+        assert isinstance(hamil, Hamiltonian)
+        self._check_number(hamil)
+        if inplace and not hamil.quadratic():
+            raise ValueError("Inplace is not implemented for this case")
+        work = self if inplace else copy.deepcopy(self)
+        final = work.apply_generated_unitary(time, 'taylor', hamil)
+        if numpy.abs(hamil.e_0()) > 1.0e-15:
+            final.scale(numpy.exp(-1.j * time * hamil.e_0()))
+        return final
+
+    def apply_generated_unitary(self, time: float, algo: str, hamil,
+                                accuracy: float = 1.0E-15,
+                                expansion: int = 30,
+                                spec_lim: Optional[List[float]] = None
+                                ) -> 'Wavefunction':
+        """Taylor propagation (reference ``src/fqe/wavefunction.py:509-611``):
+        psi(t) = sum_k (-i t H)^k psi / k!, stopping when
+        ||(-i t H)^k psi|| / k! < accuracy, else RuntimeError after
+        expansion-1 terms.  The whole series runs on the device: each term
+        is one fused apply whose epilogue scales by -i t / k, adds the term
+        to the FP64 sum and reduces its norm; a device flag ends the series
+        without host round trips."""
+        hamil = build_hamiltonian(hamil, norb=self.norb(),
+                                  conserve_number=self.conserve_number())
+        assert isinstance(hamil, Hamiltonian)
+        if not isinstance(expansion, int):
+            raise TypeError(
+                "expansion must be an int. You provided {}".format(expansion))
+        assert algo in ['taylor', 'chebyshev']
+        if algo == 'chebyshev':
+            from .chebyshev import chebyshev_evolve
+            out = chebyshev_evolve(self, time, hamil, accuracy, expansion,
+                                   spec_lim)
+        else:
+            out = _taylor(self, time, hamil, accuracy, expansion)
+        if numpy.abs(hamil.e_0() * time) > 1.e-15:
+            out.scale(numpy.exp(-1.j * time * hamil.e_0()))
+        return out
+
+    def print_wfn(self, threshold: float = 0.001, fmt: str = 'str') -> None:
+        for key, sec in self._civec.items():
+            graph = sec.get_fcigraph()
+            coeff = sec.coeff
+            print('Sector N = {} : S_z = {}'.format(key[0], key[1]))
+            for ia in range(sec.lena()):
+                for ib in range(sec.lenb()):
+                    if numpy.abs(coeff[ia, ib]) > threshold:
+                        a = int(graph.string_alpha(ia))
+                        b = int(graph.string_beta(ib))
+                        print("a'{}'b'{}' {}".format(
+                            format(a, 'b').zfill(self._norb),
+                            format(b, 'b').zfill(self._norb), coeff[ia, ib]))
+
+
+def _prepared(hamil: Optional[RestrictedHamiltonian], array,
+              norb: int) -> engine.PreparedOperator:
+    dev = _lib.device()
+    if hamil is None:
+        return engine.prepare_operator(array, norb, dev)
+    key = ("op", dev.index)
+    if key not in hamil._prepared:
+        hamil._prepared[key] = engine.prepare_operator(hamil.tensors(), norb,
+                                                       dev)
+    return hamil._prepared[key]
+
+
+def _taylor(wfn: Wavefunction, time: float, hamil: RestrictedHamiltonian,
+            accuracy: float, expansion: int) -> Wavefunction:
+    op = _prepared(hamil, hamil.tensors(), wfn.norb())
+    dev = _lib.device()
+    out = copy.deepcopy(wfn)
+    secs = list(out._civec.values())
+    if len(secs) == 1:
+        return _taylor_single(out, secs[0], op, time, accuracy, expansion)
+    # several sectors: one device flag and one norm over all of them, as the
+    # reference's loop tests the norm of the whole wavefunction
+    terms, results, structs, wss, offs = [], [], [], [], [0]
+    for sec in secs:
+        c = sec.device_coeff()
+        terms.append(c.clone())
+        results.append(torch.view_as_real(c.to(torch.complex128).clone()))
+        s, keep = engine.sector_struct(sec.get_fcigraph(), op)
+        structs.append((s, keep))
+        wss.append(engine.workspace(s, engine.precision_of(c.dtype), dev))
+        offs.append(offs[-1] + engine.norm_partials_count(c.numel()))
+    partials = torch.zeros(max(offs[-1], 1), dtype=torch.float64, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+    norm = torch.zeros(1, dtype=torch.float64, device=dev)
+    opc = op.c_struct()
+    for order in range(1, expansion):
+        alpha = -1.0j * time / order
+        for k, sec in enumerate(secs):
+            prec = engine.precision_of(terms[k].dtype)
+            if op.h1 is not None:
+                nxt = torch.empty_like(terms[k])
+                _lib.call("fqehip_apply1", prec, op.npair, sec.lena(),
+                          sec.lenb(), structs[k][0].ptab_a,
+                          structs[k][0].ptab_b, _lib.ptr(op.h1),
+                          _lib.ptr(terms[k]), 0.0, alpha.imag, 0.0, 0.0, None,
+                          _lib.ptr(nxt), _lib.ptr(results[k]),
+                          _lib.ptr(partials[offs[k]:]), _lib.ptr(status[1:]),
+                          _lib.stream_ptr())
+                terms[k] = nxt
+            else:
+                _lib.call("fqehip_apply", prec, engine._as_ref(structs[k][0]),
+                          engine._as_ref(opc), _lib.ptr(terms[k]), 0.0,
+                          alpha.imag, 0.0, 0.0, None, _lib.ptr(terms[k]),
+                          _lib.ptr(results[k]), _lib.ptr(partials[offs[k]:]),
+                          _lib.ptr(status[1:]), _lib.ptr(wss[k]),
+                          _lib.stream_ptr())
+        _lib.call("fqehip_norm_finalize", _lib.ptr(partials), offs[-1],
+                  _lib.ptr(norm), accuracy, order, _lib.ptr(status[1:]),
+                  _lib.ptr(status[:1]), _lib.stream_ptr())
+    st = status.cpu().numpy()
+    if expansion <= 1 or st[1] == 0:
+        raise RuntimeError("maximum taylor expansion limit reached")
+    for k, sec in enumerate(secs):
+        sec.set_device_coeff(torch.view_as_complex(results[k]))
+    out._taylor_terms = int(st[0])
+    return out
+
+
+def _taylor_single(out: Wavefunction, sec: FqeData,
+                   op: engine.PreparedOperator, time: float, accuracy: float,
+                   expansion: int) -> Wavefunction:
+    dev = _lib.device()
+    c = sec.device_coeff()
+    prec = engine.precision_of(c.dtype)
+    term = c.clone()
+    result = c.to(torch.complex128).clone()
+    s, keep = engine.sector_struct(sec.get_fcigraph(), op)
+    ws = engine.workspace(s, prec, dev)
+    partials = torch.zeros(max(engine.norm_partials_count(c.numel()), 1),
+                           dtype=torch.float64, device=dev)
+    status = torch.zeros(2, dtype=torch.int32, device=dev)
+    norm = torch.zeros(1, dtype=torch.float64, device=dev)
+    opc = op.c_struct()
+    _lib.call("fqehip_taylor", prec, engine._as_ref(s), engine._as_ref(opc),
+              float(time), float(accuracy), int(expansion), _lib.ptr(term),
+              _lib.ptr(result), _lib.ptr(status), _lib.ptr(partials),
+              _lib.ptr(norm), _lib.ptr(ws), _lib.stream_ptr())
+    st = status.cpu().numpy()
+    del keep
+    if expansion <= 1 or st[1] == 0:
+        raise RuntimeError("maximum taylor expansion limit reached")
+    sec.set_device_coeff(result)
+    out._taylor_terms = int(st[0])
+    return out

@@ -1,0 +1,345 @@
+"""GPU parity of the HIP engine against the oracle and the reference's
+golden data.  Every test runs the HIP kernels through the C ABI.
+
+Tolerances: FP64 engine results agree with the complex128 oracle to a
+normwise relative error of 1e-12 (tables bit-exact); the mixed engine
+(complex64 storage, BF16 hi/lo matrix-core GEMM) to 2e-4.
+"""
+
+import copy
+import json
+import os
+
+import numpy
+import pytest
+
+from oracle import fqe_oracle as fo
+from oracle import ref_lm
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+FP64_TOL = 1e-12
+MIXED_TOL = 2e-4
+
+
+@pytest.fixture(scope="module")
+def fqe():
+    import torch
+    assert torch.cuda.is_available()
+    import fqe_amd
+    return fqe_amd
+
+
+def rel(a, b):
+    return numpy.linalg.norm(numpy.asarray(a) - numpy.asarray(b)) / max(
+        numpy.linalg.norm(numpy.asarray(b)), 1e-300)
+
+
+# --------------------------------------------------------------------------
+# (a) tables
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("norb,nele", [(1, 0), (1, 1), (4, 2), (6, 3),
+                                       (8, 4), (7, 0), (7, 7), (9, 2),
+                                       (10, 5)])
+def test_string_tables_bit_exact(fqe, norb, nele):
+    print(f"tables norb={norb} nele={nele}", flush=True)
+    g = fqe.FciGraph(nele, 0, norb)
+    ref = fo.OracleGraph(nele, 0, norb)
+    assert numpy.array_equal(g._astr, ref.astr)
+    assert g._aind == {int(s): a for a, s in enumerate(ref.astr)}
+    if nele:
+        assert numpy.array_equal(g._dexca, ref.dexca)
+    for key, val in ref.alpha_map.items():
+        assert numpy.array_equal(g._alpha_map[key], val), key
+
+
+def test_string_tables_match_reference_literal(fqe):
+    with open(os.path.join(GOLDEN, "literal_tables.json")) as f:
+        lit = json.load(f)
+    g = fqe.FciGraph(4, 0, 8)
+    assert g._astr.tolist() == lit["strings_n8_k4"]
+    assert {str(k): v for k, v in g._aind.items()} == lit["index_n8_k4"]
+    g = fqe.FciGraph(2, 1, 4)
+    for key, rows in lit["maps_n4_k2"].items():
+        i, j = map(int, key.split(","))
+        assert g.alpha_map(i, j).tolist() == [list(r) for r in rows]
+    for key, rows in lit["maps_n4_k1"].items():
+        i, j = map(int, key.split(","))
+        assert g.beta_map(i, j).tolist() == [list(r) for r in rows]
+
+
+@pytest.mark.parametrize("compressed", [False, True])
+def test_pair_table(fqe, compressed):
+    norb, nele = 8, 3
+    g = fqe.FciGraph(nele, 0, norb)
+    tab = g.tables(True).pair_table(compressed).cpu().numpy()
+    ref = fo.OracleGraph(nele, 0, norb)
+    npair = norb * (norb + 1) // 2 if compressed else norb * norb
+    expect = numpy.zeros((npair, ref.lena), dtype=numpy.int32)
+    for t in range(ref.lena):
+        for src, ij, sign in ref.dexca[t]:
+            i, j = divmod(int(ij), norb)
+            p = (min(i, j) + max(i, j) * (max(i, j) + 1) // 2
+                 if compressed else ij)
+            assert expect[p, t] == 0
+            expect[p, t] = sign * (src + 1)
+    assert numpy.array_equal(tab, expect)
+
+
+# --------------------------------------------------------------------------
+# (b) dvec
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("na,nb,norb", [(2, 3, 6), (3, 3, 6), (0, 2, 5),
+                                        (4, 1, 7)])
+def test_dvec_matches_oracle(fqe, na, nb, norb):
+    rng = numpy.random.default_rng(na + 10 * nb + 100 * norb)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    d = fqe.FqeData(na, nb, norb)
+    d.coeff = c
+    assert rel(d.calculate_dvec_spatial(), fo.dvec(c, g)) < FP64_TOL
+
+
+# --------------------------------------------------------------------------
+# full apply
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("na,nb,norb", [(2, 3, 6), (2, 1, 4), (1, 1, 2)])
+def test_apply_matches_fqe_golden(fqe, na, nb, norb):
+    """FQE's own outputs (tests/unittest_data/fqe_data) for apply((h1,)),
+    apply((0, h2)), apply((h1, h2)) and the complex variant of
+    tests/fqe_data_test.py:541-568."""
+    z = numpy.load(os.path.join(GOLDEN, "fqe_data_golden.npz"))
+    tag = f"{na:02d}{nb:02d}{norb:02d}"
+    c, h1, h2 = z["c_" + tag], z["h1_" + tag], z["h2_" + tag]
+    d = fqe.FqeData(na, nb, norb)
+    d.coeff = c
+    assert rel(d.apply((h1,)).coeff, z["out1_" + tag]) < FP64_TOL
+    assert rel(d.apply((numpy.zeros_like(h1), h2)).coeff,
+               z["out2_" + tag]) < FP64_TOL
+    assert rel(d.apply((h1, h2)).coeff, z["out12_" + tag]) < FP64_TOL
+    h1c = h1.astype(numpy.complex128) + 1e-7j
+    out = d.apply((h1c, h2.astype(numpy.complex128))).coeff
+    assert numpy.allclose(out, z["out12_" + tag])
+    g = fo.OracleGraph(na, nb, norb)
+    assert rel(out, fo.apply_12(c, h1c, h2, g)) < FP64_TOL
+
+
+def test_apply_closed_form_eigenvector(fqe):
+    """tests/fqe_data_test.py:761-826"""
+    with open(os.path.join(GOLDEN, "literal_tables.json")) as f:
+        lit = json.load(f)
+    norb = 4
+    h2 = numpy.zeros((norb,) * 4, dtype=numpy.complex128)
+    for i, j, k, l in numpy.ndindex(h2.shape):
+        h2[i, j, k, l] = (i + k) * (j + l) * 0.02
+    wfn = numpy.array(lit["eigvec_2_2_4"])
+    wfn = wfn[..., 0] + 1j * wfn[..., 1]
+    d = fqe.FqeData(2, 2, norb)
+    d.coeff = wfn
+    out = d.apply((numpy.zeros((norb, norb), dtype=numpy.complex128), h2))
+    assert numpy.allclose(out.coeff, lit["eigval_2_2_4"] * wfn, atol=1e-12)
+
+
+def _h(norb, rng, kind):
+    h1 = rng.normal(size=(norb, norb))
+    h2 = rng.normal(size=(norb,) * 4)
+    if kind == "complex":
+        h1 = h1 + 1j * rng.normal(size=(norb, norb))
+        h2 = h2 + 1j * rng.normal(size=(norb,) * 4)
+    elif kind == "symmetric":
+        return fo.synthetic_hamiltonian(norb, norb, rng)
+    return h1, h2
+
+
+@pytest.mark.parametrize("kind", ["real", "complex", "symmetric"])
+@pytest.mark.parametrize("na,nb,norb", [(2, 2, 4), (3, 2, 6), (0, 3, 5),
+                                        (4, 4, 4), (3, 4, 8)])
+def test_apply_matches_oracle(fqe, kind, na, nb, norb):
+    print(f"apply {kind} {na} {nb} {norb}", flush=True)
+    rng = numpy.random.default_rng(norb * 31 + na * 7 + nb)
+    h1, h2 = _h(norb, rng, kind)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    d = fqe.FqeData(na, nb, norb)
+    d.coeff = c
+    assert rel(d.apply((h1, h2)).coeff, fo.apply_12(c, h1, h2, g)) < FP64_TOL
+    assert rel(d.apply((h1,)).coeff, fo.apply_1(c, h1, g)) < FP64_TOL
+
+
+def test_wavefunction_apply_with_e0_multi_sector(fqe):
+    norb = 5
+    rng = numpy.random.default_rng(11)
+    h1, h2 = fo.synthetic_hamiltonian(norb, 3, rng)
+    e0 = 0.7 - 0.2j
+    wfn = fqe.Wavefunction([[4, 0, norb], [4, 2, norb], [4, -2, norb]])
+    wfn.set_wfn(strategy="random")
+    ham = fqe.get_restricted_hamiltonian((h1, h2), e_0=e0)
+    out = fqe.apply(ham, wfn)
+    for key in wfn.sectors():
+        sec = wfn.sector(key)
+        g = fo.OracleGraph(sec.nalpha(), sec.nbeta(), norb)
+        ref = fo.apply_hamiltonian(sec.coeff, (h1, h2), e0, g)
+        assert rel(out.sector(key).coeff, ref) < FP64_TOL
+    # expectation value and Hermiticity
+    ev = wfn.expectationValue(ham)
+    other = copy.deepcopy(wfn)
+    other.set_wfn(strategy="random")
+    lhs = fqe.vdot(other, wfn.apply(ham))
+    rhs = numpy.conj(fqe.vdot(wfn, other.apply(ham)))
+    assert abs(lhs - rhs - (e0 - numpy.conj(e0)) * fqe.vdot(other, wfn)) < 1e-12
+    assert abs(ev.imag - e0.imag) < 1e-12
+
+
+def test_apply_errors(fqe):
+    wfn = fqe.Wavefunction([[2, 0, 3]])
+    with pytest.raises(ValueError):
+        wfn.apply(fqe.get_restricted_hamiltonian(
+            (numpy.zeros((4, 4)), numpy.zeros((4,) * 4))))
+    ham = fqe.get_restricted_hamiltonian((numpy.eye(3),
+                                          numpy.zeros((3,) * 4)))
+    ham._conserve_number = False
+    with pytest.raises(TypeError):
+        wfn.apply(ham)
+
+
+# --------------------------------------------------------------------------
+# Taylor time evolution
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("na,nb,norb,e0", [(2, 2, 4, 0.0), (3, 2, 6, 0.4),
+                                           (4, 4, 8, 0.0)])
+def test_time_evolve_matches_oracle(fqe, na, nb, norb, e0):
+    rng = numpy.random.default_rng(norb + na)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    ref, nterms = fo.taylor_evolve(c, (h1, h2), e0, 0.1, g)
+    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]])
+    wfn.set_wfn(strategy="from_data", raw_data={(na + nb, na - nb): c})
+    ham = fqe.get_restricted_hamiltonian((h1, h2), e_0=e0)
+    out = wfn.time_evolve(0.1, ham)
+    assert rel(out.get_coeff((na + nb, na - nb)), ref) < FP64_TOL
+    assert out._taylor_terms == nterms
+
+
+def test_time_evolve_multi_sector_shares_term_count(fqe):
+    norb = 4
+    rng = numpy.random.default_rng(2)
+    h1, h2 = fo.synthetic_hamiltonian(norb, 2, rng)
+    wfn = fqe.Wavefunction([[2, 0, norb], [2, 2, norb]])
+    wfn.set_wfn(strategy="random")
+    out = wfn.time_evolve(0.2, fqe.get_restricted_hamiltonian((h1, h2)))
+    # the oracle on the stacked sectors: norms add over sectors
+    totals = []
+    for key in wfn.sectors():
+        sec = wfn.sector(key)
+        g = fo.OracleGraph(sec.nalpha(), sec.nbeta(), norb)
+        totals.append((key, sec.coeff, g))
+    iht = (-0.2j * h1, -0.2j * h2)
+    work = {k: c.copy() for k, c, _ in totals}
+    res = {k: c.copy() for k, c, _ in totals}
+    import math
+    for order in range(1, 30):
+        nrm = 0.0
+        for k, _, g in totals:
+            work[k] = fo.apply_12(work[k], iht[0], iht[1], g)
+            res[k] = res[k] + work[k] / math.factorial(order)
+            nrm += numpy.linalg.norm(work[k]) ** 2
+        if math.sqrt(nrm) / math.factorial(order) < 1e-15:
+            break
+    assert out._taylor_terms == order
+    for k, _, _ in totals:
+        assert rel(out.get_coeff(k), res[k]) < FP64_TOL
+
+
+def test_time_evolve_not_converged_raises(fqe):
+    wfn = fqe.Wavefunction([[2, 0, 3]])
+    wfn.set_wfn(strategy="random")
+    rng = numpy.random.default_rng(0)
+    h1, h2 = fo.synthetic_hamiltonian(3, 2, rng)
+    with pytest.raises(RuntimeError):
+        wfn.apply_generated_unitary(1.0, 'taylor',
+                                    fqe.get_restricted_hamiltonian((h1, h2)),
+                                    expansion=3)
+
+
+def test_time_evolve_quadratic(fqe):
+    norb, na, nb = 5, 2, 2
+    rng = numpy.random.default_rng(9)
+    h1, _ = fo.synthetic_hamiltonian(norb, 1, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    wfn = fqe.Wavefunction([[4, 0, norb]])
+    wfn.set_wfn(strategy="from_data", raw_data={(4, 0): c})
+    out = wfn.time_evolve(0.1, fqe.get_restricted_hamiltonian((h1,)))
+    dense = numpy.zeros((g.lena * g.lenb,) * 2, dtype=complex)
+    for col in range(dense.shape[0]):
+        e = numpy.zeros(dense.shape[0], dtype=complex)
+        e[col] = 1
+        dense[:, col] = fo.apply_1(e.reshape(g.lena, g.lenb), h1, g).ravel()
+    import scipy.linalg
+    exact = (scipy.linalg.expm(-0.1j * dense) @ c.ravel()).reshape(c.shape)
+    assert rel(out.get_coeff((4, 0)), exact) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# the plain-method workloads
+# --------------------------------------------------------------------------
+
+def test_small_workload_apply_and_evolve(fqe):
+    """norb=8, 4a4b, seed 0, 8 factors: one apply + time_evolve(0.1)."""
+    c, h1, h2 = fo.synthetic_workload(8, 4, 4, 0, 8)
+    g = fo.OracleGraph(4, 4, 8)
+    wfn = fqe.Wavefunction([[8, 0, 8]])
+    wfn.set_wfn(strategy="from_data", raw_data={(8, 0): c})
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    out = wfn.apply(ham)
+    assert rel(out.get_coeff((8, 0)), fo.apply_12(c, h1, h2, g)) < FP64_TOL
+    if ref_lm.available():
+        assert rel(out.get_coeff((8, 0)),
+                   ref_lm.apply_12_lm(c, h1, h2, g)) < FP64_TOL
+    ref, nterms = fo.taylor_evolve(c, (h1, h2), 0.0, 0.1, g)
+    ev = wfn.time_evolve(0.1, ham)
+    assert rel(ev.get_coeff((8, 0)), ref) < FP64_TOL
+    assert ev._taylor_terms == nterms
+
+
+@pytest.mark.skipif(not ref_lm.available(), reason="oracle/_ref not built")
+def test_production_size_apply_vs_reference_c(fqe):
+    """norb=12, 6a6b (853,776 determinants), seed 1, 12 factors: the
+    engine's apply against the reference's own compiled lm kernels."""
+    print("norb=12 apply vs reference C", flush=True)
+    c, h1, h2 = fo.synthetic_workload(12, 6, 6, 1, 12)
+    d = fqe.FqeData(6, 6, 12)
+    d.coeff = c
+    out = d.apply((h1, h2)).coeff
+    rg = ref_lm.RefGraph(6, 6, 12)
+    ref = ref_lm.apply_12_lm(c, h1, h2, rg)
+    assert rel(out, ref) < FP64_TOL
+
+
+# --------------------------------------------------------------------------
+# mixed precision
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("na,nb,norb", [(3, 3, 6), (4, 4, 8)])
+def test_mixed_precision_apply_and_evolve(fqe, na, nb, norb):
+    c, h1, h2 = fo.synthetic_workload(norb, na, nb, 5, norb)
+    g = fo.OracleGraph(na, nb, norb)
+    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]], dtype=numpy.complex64)
+    key = (na + nb, na - nb)
+    wfn.set_wfn(strategy="from_data", raw_data={key: c})
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    out = wfn.apply(ham).get_coeff(key)
+    assert out.dtype == numpy.complex64
+    err = rel(out, fo.apply_12(c, h1, h2, g))
+    print(f"mixed apply rel err {err:.3e}", flush=True)
+    assert err < MIXED_TOL
+    ref, _ = fo.taylor_evolve(c, (h1, h2), 0.0, 0.1, g)
+    err = rel(wfn.time_evolve(0.1, ham).get_coeff(key), ref)
+    print(f"mixed evolve rel err {err:.3e}", flush=True)
+    assert err < MIXED_TOL
