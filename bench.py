@@ -1,0 +1,265 @@
+"""Benchmark of the RestrictedHamiltonian apply on one MI355X.
+
+Workload (the headline configuration): norb=14, 7 alpha + 7 beta electrons
+(3,432 x 3,432 = 11,778,624 determinants), complex coefficients with real
+and imaginary parts i.i.d. N(0,1) from seed 2, normalised; h1/h2 from 14
+Cholesky-like factors (oracle.fqe_oracle.synthetic_workload).  A step is
+one apply H|C>.
+
+The engine runs it twice: FP64 (complex128 storage, FP64 MFMA GEMM) and
+mixed precision (complex64 storage, BF16 hi/lo split MFMA GEMM with FP32
+accumulation).  ``value`` is the mixed engine's applies/s; the FP64 rate,
+the speed-up and the relative errors (mixed vs FP64 on the GPU, both vs
+the reference's complex128 apply) are reported beside it.
+
+Per-kernel durations are measured with HIP events on the stream the
+kernels run on; ``roofline`` is given for the dominant kernel of the
+mixed engine.  ``cpu_baseline`` times the reference's own compiled C
+(lm kernels, complex128, oracle/_ref) on the host cores for one apply.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "openfermion-fqe_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = ("applies/sec (norb=14,7α7β): mixed vs FP64 on GPU at "
+          "≤1e-9 rel err")
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"f64": 78.6, "bf16": 2500.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--norb", type=int, default=14)
+    ap.add_argument("--nalpha", type=int, default=7)
+    ap.add_argument("--nbeta", type=int, default=7)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+class EngineRun:
+    """Device buffers and the three kernel launches of one apply."""
+
+    def __init__(self, fqe, graph, op, coeff_np, torch_dtype):
+        import torch
+        from fqe_amd import _lib, engine
+        self.torch = torch
+        self._lib = _lib
+        self.engine = engine
+        self.graph = graph
+        self.op = op
+        dev = _lib.device()
+        self.c = torch.from_numpy(coeff_np).to(dev).to(torch_dtype)
+        self.out = torch.empty_like(self.c)
+        self.prec = engine.precision_of(torch_dtype)
+        self.s, self.keep = engine.sector_struct(graph, op)
+        self.ws = engine.workspace(self.s, self.prec, dev)
+        self.n = self.c.numel()
+        esz = 8 if self.prec == _lib.C128 else 4
+        self.dvec = self.ws
+        self.y_ptr = self.ws.data_ptr() + op.npair * 2 * self.n * esz
+
+    def launch(self, events=None):
+        L, s, op = self._lib, self.s, self.op
+        st = L.stream_ptr()
+        if events:
+            events[0].record()
+        L.call("fqehip_dvec_gather", self.prec, op.npair, s.lena, s.lenb,
+               s.ptab_a, s.ptab_b, L.ptr(self.c), L.ptr(self.dvec), st)
+        if events:
+            events[1].record()
+        if op.complex_op:
+            L.call("fqehip_pair_gemm", self.prec, 2 * (op.npair + 1),
+                   2 * op.npair, L.ptr(op.a), self.n, L.ptr(self.dvec),
+                   self.y_ptr, st)
+        else:
+            L.call("fqehip_pair_gemm", self.prec, op.npair + 1, op.npair,
+                   L.ptr(op.a), 2 * self.n, L.ptr(self.dvec), self.y_ptr, st)
+        if events:
+            events[2].record()
+        L.call("fqehip_scatter", self.prec, op.npair, s.lena, s.lenb,
+               s.ptab_a, s.ptab_b, self.y_ptr, 1.0, 0.0, 0.0, 0.0, None,
+               L.ptr(self.out), None, None, None, st)
+        if events:
+            events[3].record()
+
+    def run(self, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            self.launch()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)]
+               for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            self.launch(evs[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        names = ["dvec_gather", "pair_gemm", "scatter"]
+        per = {nm: sum(e[i].elapsed_time(e[i + 1]) for e in evs) / steps
+               for i, nm in enumerate(names)}
+        return wall, per
+
+    def result(self):
+        return self.out.to(self.torch.complex128).cpu().numpy()
+
+    def free(self):
+        del self.c, self.out, self.ws, self.dvec
+        self.torch.cuda.empty_cache()
+
+
+def kernel_work(op, n, esz):
+    """Algorithmic bytes / flops of one launch of each kernel."""
+    p = op.npair
+    return {
+        "dvec_gather": {"bytes": (p + 1) * 2 * n * esz},
+        "pair_gemm": {"bytes": (2 * p + 1) * 2 * n * esz,
+                      "flops": 2.0 * (p + 1) * p * 2 * n},
+        "scatter": {"bytes": (p + 1 + 1) * 2 * n * esz},
+    }
+
+
+def roofline_entry(name, per_ms, work, mode):
+    t = per_ms[name] * 1e-3
+    w = work[name]
+    if name == "pair_gemm" and mode == "fp64":
+        ach = w["flops"] / t / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": round(ach, 2),
+                "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
+                "frac": round(ach / PEAK_TFLOPS["f64"], 4), "traffic": None}
+    ach = w["bytes"] / t / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+
+
+def cpu_baseline(coeff, h1, h2, norb, na, nb):
+    """One apply with the reference's compiled lm kernels on all host
+    cores (complex128)."""
+    from oracle import ref_lm
+    if not ref_lm.available():
+        return None, None
+    import platform
+    cores = len(os.sched_getaffinity(0))
+    graph = ref_lm.RefGraph(na, nb, norb)
+    t0 = time.perf_counter()
+    out = ref_lm.apply_12_lm(coeff, h1, h2, graph)
+    dt = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(1.0 / dt, 5), "unit": "applies/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+            "kind": "reference",
+            "sample": (f"1 apply of the same workload (norb={norb}, "
+                       f"{na}a{nb}b) by the reference's compiled C lm "
+                       f"kernels in complex128 (OpenMP, scipy BLAS zaxpy), "
+                       f"{dt:.2f} s; host CPU {cpu} "
+                       f"({platform.machine()})")}, out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not "
+              f"supported in this build (multi-GPU sharding is designed, not "
+              f"built)", file=sys.stderr)
+        sys.exit(2)
+    import numpy
+    import torch
+    import fqe_amd as fqe
+    from fqe_amd import engine, _lib
+    from oracle import fqe_oracle as fo
+
+    norb, na, nb = args.norb, args.nalpha, args.nbeta
+    coeff, h1, h2 = fo.synthetic_workload(norb, na, nb, args.seed, norb)
+    graph = fqe.FciGraph(na, nb, norb)
+    dev = _lib.device()
+    op = engine.prepare_operator((h1, h2), norb, dev)
+    n = coeff.size
+
+    results = {}
+    outs = {}
+    for mode, dt in (("fp64", torch.complex128), ("mixed", torch.complex64)):
+        run = EngineRun(fqe, graph, op, coeff, dt)
+        wall, per = run.run(args.steps, args.warmup)
+        outs[mode] = run.result()
+        results[mode] = {"wall": wall, "per": per,
+                         "esz": 8 if dt == torch.complex128 else 4}
+        run.free()
+        del run
+
+    def rel(a, b):
+        return float(numpy.linalg.norm(a - b) / numpy.linalg.norm(b))
+
+    err_mixed_fp64 = rel(outs["mixed"], outs["fp64"])
+
+    cpu, ref_out = (None, None)
+    if not args.no_cpu_baseline:
+        cpu, ref_out = cpu_baseline(coeff, h1, h2, norb, na, nb)
+
+    mixed = results["mixed"]
+    fp64 = results["fp64"]
+    value = args.steps / mixed["wall"]
+    fp64_value = args.steps / fp64["wall"]
+    work_m = kernel_work(op, n, 4)
+    work_d = kernel_work(op, n, 8)
+    dom_m = max(mixed["per"], key=mixed["per"].get)
+    dom_d = max(fp64["per"], key=fp64["per"].get)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * mixed["wall"] / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 storage, bf16x3 MFMA, f32 accumulate",
+        "data": "synthetic",
+        "config": {"workload": (f"RestrictedHamiltonian apply, norb={norb} "
+                                f"{na}a{nb}b ({n} determinants), seed "
+                                f"{args.seed}, {norb} factors"),
+                   "pair_basis": "compressed" if op.compressed else "ordered",
+                   "npair": op.npair},
+        "fp64": {"value": round(fp64_value, 3), "unit": "applies/s",
+                 "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3),
+                 "kernels_ms": {k: round(v, 4) for k, v in fp64["per"].items()},
+                 "roofline": roofline_entry(dom_d, fp64["per"], work_d,
+                                            "fp64")},
+        "speedup_mixed_vs_fp64": round(value / fp64_value, 3),
+        "rel_err_mixed_vs_fp64_gpu": err_mixed_fp64,
+        "kernels_ms": {k: round(v, 4) for k, v in mixed["per"].items()},
+        "roofline": roofline_entry(dom_m, mixed["per"], work_m, "mixed"),
+        "cpu_baseline": cpu,
+    }
+    if ref_out is not None:
+        line["rel_err_fp64_vs_reference"] = rel(outs["fp64"], ref_out)
+        line["rel_err_mixed_vs_reference"] = rel(outs["mixed"], ref_out)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
