@@ -47,47 +47,75 @@ def parse():
     ap.add_argument("--nbeta", type=int, default=7)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--modes", default="fp64,mixed",
+                    help="engines to run (profiling runs use one)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="first copy this many bytes device-to-device (the "
+                         "calibration kernel of HBM traffic counters)")
+    ap.add_argument("--unfused", action="store_true",
+                    help="time the unfused gather / GEMM / scatter kernels")
     return ap.parse_args()
 
 
 class EngineRun:
-    """Device buffers and the three kernel launches of one apply."""
+    """Device buffers and the kernel launches of one apply (fused path when
+    the engine supports the sector, else gather / GEMM / scatter)."""
 
-    def __init__(self, fqe, graph, op, coeff_np, torch_dtype):
+    def __init__(self, graph, op, coeff_np, torch_dtype, fused=True):
+        import ctypes
         import torch
-        from fqe_amd import _lib, engine
+        from fqe_amd import _lib, engine, settings
         self.torch = torch
         self._lib = _lib
-        self.engine = engine
         self.graph = graph
         self.op = op
         dev = _lib.device()
         self.c = torch.from_numpy(coeff_np).to(dev).to(torch_dtype)
         self.out = torch.empty_like(self.c)
         self.prec = engine.precision_of(torch_dtype)
+        saved = settings.fused
+        settings.fused = fused
         self.s, self.keep = engine.sector_struct(graph, op)
+        settings.fused = saved
+        self.opc = op.c_struct()
+        self.sref = ctypes.byref(self.s)
+        self.oref = ctypes.byref(self.opc)
+        self.fused = bool(_lib.lib().fqehip_fused_supported(
+            self.prec, self.sref, self.oref))
         self.ws = engine.workspace(self.s, self.prec, dev)
         self.n = self.c.numel()
         esz = 8 if self.prec == _lib.C128 else 4
-        self.dvec = self.ws
         self.y_ptr = self.ws.data_ptr() + op.npair * 2 * self.n * esz
+        self.names = (["fused_rows", "slot_reduce"] if self.fused else
+                      ["dvec_gather", "pair_gemm", "scatter"])
 
     def launch(self, events=None):
         L, s, op = self._lib, self.s, self.op
         st = L.stream_ptr()
         if events:
             events[0].record()
+        if self.fused:
+            L.call("fqehip_apply_fused_rows", self.prec, self.sref, self.oref,
+                   L.ptr(self.c), L.ptr(self.ws), None, st)
+            if events:
+                events[1].record()
+            L.call("fqehip_apply_slot_reduce", self.prec, self.sref,
+                   L.ptr(self.ws), 1.0, 0.0, 0.0, 0.0, None, L.ptr(self.out),
+                   None, None, None, st)
+            if events:
+                events[2].record()
+            return
         L.call("fqehip_dvec_gather", self.prec, op.npair, s.lena, s.lenb,
-               s.ptab_a, s.ptab_b, L.ptr(self.c), L.ptr(self.dvec), st)
+               s.ptab_a, s.ptab_b, L.ptr(self.c), L.ptr(self.ws), st)
         if events:
             events[1].record()
         if op.complex_op:
             L.call("fqehip_pair_gemm", self.prec, 2 * (op.npair + 1),
-                   2 * op.npair, L.ptr(op.a), self.n, L.ptr(self.dvec),
+                   2 * op.npair, L.ptr(op.a), self.n, L.ptr(self.ws),
                    self.y_ptr, st)
         else:
             L.call("fqehip_pair_gemm", self.prec, op.npair + 1, op.npair,
-                   L.ptr(op.a), 2 * self.n, L.ptr(self.dvec), self.y_ptr, st)
+                   L.ptr(op.a), 2 * self.n, L.ptr(self.ws), self.y_ptr, st)
         if events:
             events[2].record()
         L.call("fqehip_scatter", self.prec, op.npair, s.lena, s.lenb,
@@ -101,7 +129,8 @@ class EngineRun:
         for _ in range(warmup):
             self.launch()
         torch.cuda.synchronize()
-        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        nev = len(self.names) + 1
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(nev)]
                for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -109,42 +138,54 @@ class EngineRun:
             self.launch(evs[k])
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        names = ["dvec_gather", "pair_gemm", "scatter"]
         per = {nm: sum(e[i].elapsed_time(e[i + 1]) for e in evs) / steps
-               for i, nm in enumerate(names)}
+               for i, nm in enumerate(self.names)}
         return wall, per
 
     def result(self):
         return self.out.to(self.torch.complex128).cpu().numpy()
 
     def free(self):
-        del self.c, self.out, self.ws, self.dvec
+        del self.c, self.out, self.ws
         self.torch.cuda.empty_cache()
 
 
-def kernel_work(op, n, esz):
-    """Algorithmic bytes / flops of one launch of each kernel."""
+def kernel_work(op, graph, n, esz, mixed):
+    """Algorithmic bytes and flops of one launch of each kernel.  GEMM
+    flops are those of the real (npair+1) x npair pair matrix on 2n real
+    columns; the mixed engine executes three BF16 products per flop."""
     p = op.npair
+    nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
+    flops = 2.0 * (p + 1) * p * 2 * n
     return {
         "dvec_gather": {"bytes": (p + 1) * 2 * n * esz},
-        "pair_gemm": {"bytes": (2 * p + 1) * 2 * n * esz,
-                      "flops": 2.0 * (p + 1) * p * 2 * n},
+        "pair_gemm": {"bytes": (2 * p + 1) * 2 * n * esz, "flops": flops},
         "scatter": {"bytes": (p + 1 + 1) * 2 * n * esz},
+        "fused_rows": {"bytes": (nslot + 2) * 2 * n * esz, "flops": flops},
+        "slot_reduce": {"bytes": (nslot + 2) * 2 * n * esz},
     }
 
 
 def roofline_entry(name, per_ms, work, mode):
+    """Roofline of one kernel: the resource whose time at peak is longer
+    bounds it (HBM bytes / 8 TB/s vs MFMA flops / peak)."""
     t = per_ms[name] * 1e-3
     w = work[name]
-    if name == "pair_gemm" and mode == "fp64":
-        ach = w["flops"] / t / 1e12
+    peak_fl = PEAK_TFLOPS["f64"] if mode == "fp64" else PEAK_TFLOPS["bf16"]
+    mult = 1.0 if mode == "fp64" else 3.0
+    t_hbm = w["bytes"] / (HBM_PEAK_GBS * 1e9)
+    t_mfma = mult * w.get("flops", 0.0) / (peak_fl * 1e12)
+    if t_mfma > t_hbm:
+        ach = mult * w["flops"] / t / 1e12
         return {"kernel": name, "bound": "mfma", "achieved": round(ach, 2),
-                "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                "frac": round(ach / PEAK_TFLOPS["f64"], 4), "traffic": None}
+                "peak": peak_fl, "unit": "TFLOP/s",
+                "frac": round(ach / peak_fl, 4), "traffic": None,
+                "flops_per_launch": mult * w["flops"]}
     ach = w["bytes"] / t / 1e9
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": w["bytes"]}
 
 
 def cpu_baseline(coeff, h1, h2, norb, na, nb):
@@ -191,6 +232,14 @@ def main():
     from fqe_amd import engine, _lib
     from oracle import fqe_oracle as fo
 
+    if args.copy_probe:
+        src = torch.empty(args.copy_probe, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}),
+              file=sys.stderr, flush=True)
+        del src, dst
     norb, na, nb = args.norb, args.nalpha, args.nbeta
     coeff, h1, h2 = fo.synthetic_workload(norb, na, nb, args.seed, norb)
     graph = fqe.FciGraph(na, nb, norb)
@@ -200,12 +249,25 @@ def main():
 
     results = {}
     outs = {}
+    modes = [m for m in ("fp64", "mixed") if m in args.modes.split(",")]
+    if modes != ["fp64", "mixed"]:
+        for mode in modes:
+            dt = torch.complex128 if mode == "fp64" else torch.complex64
+            run = EngineRun(graph, op, coeff, dt, fused=not args.unfused)
+            wall, per = run.run(args.steps, args.warmup)
+            print(json.dumps({"mode": mode, "applies_per_s": args.steps / wall,
+                              "kernels_ms": per}))
+            run.free()
+        return
     for mode, dt in (("fp64", torch.complex128), ("mixed", torch.complex64)):
-        run = EngineRun(fqe, graph, op, coeff, dt)
+        run = EngineRun(graph, op, coeff, dt, fused=not args.unfused)
         wall, per = run.run(args.steps, args.warmup)
         outs[mode] = run.result()
-        results[mode] = {"wall": wall, "per": per,
+        results[mode] = {"wall": wall, "per": per, "fused": run.fused,
                          "esz": 8 if dt == torch.complex128 else 4}
+        print(f"{mode}: {args.steps / wall:.2f} applies/s "
+              f"{ {k: round(v, 3) for k, v in per.items()} }",
+              file=sys.stderr, flush=True)
         run.free()
         del run
 
@@ -222,8 +284,8 @@ def main():
     fp64 = results["fp64"]
     value = args.steps / mixed["wall"]
     fp64_value = args.steps / fp64["wall"]
-    work_m = kernel_work(op, n, 4)
-    work_d = kernel_work(op, n, 8)
+    work_m = kernel_work(op, graph, n, 4, True)
+    work_d = kernel_work(op, graph, n, 8, False)
     dom_m = max(mixed["per"], key=mixed["per"].get)
     dom_d = max(fp64["per"], key=fp64["per"].get)
     line = {
@@ -243,7 +305,8 @@ def main():
                                 f"{na}a{nb}b ({n} determinants), seed "
                                 f"{args.seed}, {norb} factors"),
                    "pair_basis": "compressed" if op.compressed else "ordered",
-                   "npair": op.npair},
+                   "npair": op.npair,
+                   "kernels": "fused" if mixed["fused"] else "unfused"},
         "fp64": {"value": round(fp64_value, 3), "unit": "applies/s",
                  "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3),
                  "kernels_ms": {k: round(v, 4) for k, v in fp64["per"].items()},

@@ -161,6 +161,12 @@ typedef struct fqehip_sector {
   int64_t lena, lenb;
   const int32_t* ptab_a;   /* [npair][lena] */
   const int32_t* ptab_b;   /* [npair][lenb] */
+  /* Tables of the fused path (fqehip_fused_tables); NULL selects the
+   * unfused gather / GEMM / scatter kernels. */
+  const int32_t* fwd_a;    /* [npair][lena] forward table, alpha */
+  const int32_t* fwd_b;    /* [npair][lenb] forward table, beta */
+  const int32_t* slot_a;   /* [npair][lena] alpha slot index */
+  int nslot;               /* nalpha * (norb - nalpha) */
 } fqehip_sector;
 
 typedef struct fqehip_operator {
@@ -169,7 +175,40 @@ typedef struct fqehip_operator {
   const double* h1; /* one-body only apply: complex128 [npair], else NULL */
 } fqehip_operator;
 
-/* Bytes of dvec and Y workspace for one apply of a sector. */
+/* Tables of the fused apply, from a pair table (layout above):
+ *   d_fwd[p][src] = sign*(tgt+1) with <tgt|E_p|src> = sign, else 0;
+ *   d_slot[p][src] (optional) = index k of (src, p) among the off-diagonal
+ *   de-excitations of tgt, in dexc order, else -1.
+ * There is no counterpart table in the reference: its maps
+ * (src/fqe/fci_graph.py:204-244) list the same (src, tgt, sign) triples
+ * per orbital pair. */
+int fqehip_fused_tables(int norb, int nele, int64_t nstrings,
+                        const int32_t* d_dexc, int compressed,
+                        const int32_t* d_ptab, int32_t* d_fwd,
+                        int32_t* d_slot, void* stream);
+
+/* The two kernels of the fused apply, for callers that time or schedule
+ * them separately (fqehip_apply runs both).  Preconditions: real operator,
+ * fused tables set in the sector, fqehip_fused_supported() true.
+ *   fused_rows: slot rows and row accumulators into d_workspace;
+ *   slot_reduce: out = alpha * (H C) + beta * cin with the epilogue of
+ *   fqehip_scatter. */
+int fqehip_fused_supported(int precision, const fqehip_sector* s,
+                           const fqehip_operator* op);
+int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
+                            const fqehip_operator* op, const void* d_coeff,
+                            void* d_workspace, const int32_t* d_done,
+                            void* stream);
+int fqehip_apply_slot_reduce(int precision, const fqehip_sector* s,
+                             const void* d_workspace, double alpha_re,
+                             double alpha_im, double beta_re, double beta_im,
+                             const void* d_cin, void* d_out, double* d_acc,
+                             double* d_norm_partials, const int32_t* d_done,
+                             void* stream);
+
+/* Bytes of workspace for one apply of a sector: the larger of the unfused
+ * path (dvec + Y planes) and the fused path (slot rows + row
+ * accumulators). */
 int64_t fqehip_apply_workspace_bytes(int precision, const fqehip_sector* s);
 
 /* out = alpha * H C + beta * cin (see fqehip_scatter for acc/partials).

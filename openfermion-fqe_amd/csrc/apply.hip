@@ -385,6 +385,19 @@ int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
 
 }  // namespace
 
+// Defined in fused.hip.
+bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb);
+int launch_fused(int precision, int npair, int64_t lena, int64_t lenb,
+                 const double* a, const int32_t* pta, const int32_t* ptb,
+                 const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
+                 int nslot, const void* coeff, void* slots, void* rowacc,
+                 hipStream_t st, const int32_t* done);
+int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
+                       const void* slots, const void* rowacc, double ar,
+                       double ai, double br, double bi, const void* cin,
+                       void* out, double* acc, double* partials,
+                       hipStream_t st, const int32_t* done);
+
 // Declared in gemm_bf16.hip.
 int launch_gemm_bf16x3(int m, int k, const double* a, int64_t ncol,
                        const float* b, float* y, hipStream_t st,
@@ -571,11 +584,59 @@ size_t elem_size(int precision) {
 
 }  // namespace
 
+extern "C" int fqehip_fused_supported(int precision, const fqehip_sector* s,
+                                      const fqehip_operator* op) {
+  return s && op && op->a && !op->h1 && !op->complex_op && s->fwd_a &&
+         s->fwd_b && s->slot_a &&
+         fused_supported(precision, s->npair, s->lena, s->lenb);
+}
+
+extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
+                                       const fqehip_operator* op,
+                                       const void* d_coeff, void* d_workspace,
+                                       const int32_t* d_done, void* stream) {
+  clear_error();
+  if (int rc = check_sector_op(precision, s, op)) return rc;
+  FQEHIP_CHECK_ARG(fqehip_fused_supported(precision, s, op),
+                   "the fused path does not support this sector/operator");
+  FQEHIP_CHECK_ARG(d_coeff && d_workspace, "NULL pointer");
+  FQEHIP_CHECK_ARG(s->nslot == s->nalpha * (s->norb - s->nalpha),
+                   "nslot=%d != nalpha*(norb-nalpha)", s->nslot);
+  const int64_t n = s->lena * s->lenb;
+  char* ws = static_cast<char*>(d_workspace);
+  void* rowacc = ws + static_cast<int64_t>(s->nslot) * 2 * n * elem_size(precision);
+  return launch_fused(precision, s->npair, s->lena, s->lenb, op->a, s->ptab_a,
+                      s->ptab_b, s->fwd_a, s->fwd_b, s->slot_a, s->nslot,
+                      d_coeff, ws, rowacc, as_stream(stream), d_done);
+}
+
+extern "C" int fqehip_apply_slot_reduce(int precision, const fqehip_sector* s,
+                                        const void* d_workspace,
+                                        double alpha_re, double alpha_im,
+                                        double beta_re, double beta_im,
+                                        const void* d_cin, void* d_out,
+                                        double* d_acc, double* d_norm_partials,
+                                        const int32_t* d_done, void* stream) {
+  clear_error();
+  if (int rc = check_precision(precision)) return rc;
+  FQEHIP_CHECK_ARG(s && d_workspace && d_out, "NULL pointer");
+  FQEHIP_CHECK_ARG((beta_re == 0.0 && beta_im == 0.0) || d_cin,
+                   "beta != 0 needs d_cin");
+  const int64_t n = s->lena * s->lenb;
+  const char* ws = static_cast<const char*>(d_workspace);
+  const void* rowacc = ws + static_cast<int64_t>(s->nslot) * 2 * n * elem_size(precision);
+  return launch_slot_reduce(precision, s->nslot, s->lena, s->lenb, ws, rowacc,
+                            alpha_re, alpha_im, beta_re, beta_im, d_cin, d_out,
+                            d_acc, d_norm_partials, as_stream(stream), d_done);
+}
+
 extern "C" int64_t fqehip_apply_workspace_bytes(int precision,
                                                const fqehip_sector* s) {
   if (!s) return -1;
   const int64_t n = s->lena * s->lenb;
-  return static_cast<int64_t>(2 * s->npair + 1) * 2 * n *
+  const int64_t unfused = static_cast<int64_t>(2 * s->npair + 1) * 2 * n;
+  const int64_t fused = static_cast<int64_t>(s->nslot + 1) * 2 * n;
+  return (unfused > fused ? unfused : fused) *
          static_cast<int64_t>(elem_size(precision));
 }
 
@@ -597,6 +658,14 @@ extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
   FQEHIP_CHECK_ARG(d_workspace != nullptr, "NULL workspace");
   const int64_t n = s->lena * s->lenb;
   char* ws = static_cast<char*>(d_workspace);
+  if (fqehip_fused_supported(precision, s, op)) {
+    int rc = fqehip_apply_fused_rows(precision, s, op, d_coeff, d_workspace,
+                                     d_done, stream);
+    if (rc) return rc;
+    return fqehip_apply_slot_reduce(precision, s, d_workspace, alpha_re,
+                                    alpha_im, beta_re, beta_im, d_cin, d_out,
+                                    d_acc, d_norm_partials, d_done, stream);
+  }
   void* dvec = ws;
   void* y = ws + static_cast<int64_t>(s->npair) * 2 * n * elem_size(precision);
   hipStream_t st = as_stream(stream);

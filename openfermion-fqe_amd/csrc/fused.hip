@@ -1,0 +1,733 @@
+// Fused apply for real operators: one workgroup per alpha string (row Ia
+// of the CI matrix).
+//
+//   k_fused_*:  for each chunk of beta columns of row Ia
+//       1. gather the dvec tile D[q][Ia, chunk] into LDS from C
+//          (alpha part: rows C[src_a(q, Ia), chunk], coalesced; beta part:
+//          C[Ia, src_b(q, Ib)] within row Ia);
+//       2. Y tile = A D tile on the matrix cores (A = pair matrix + h1' row,
+//          held in registers for the whole workgroup);
+//       3. scatter the Y tile without atomics to global memory:
+//          - one-body row, alpha-diagonal pairs and every beta excitation
+//            land in row Ia itself: LDS accumulator row (LDS atomics);
+//          - an off-diagonal alpha excitation E_p Ia = tgt lands in a
+//            per-target "slot" row slots[tgt][k] (k = index of (Ia, p)
+//            among tgt's off-diagonal de-excitations): a plain store.
+//       The LDS row is written out as rowacc[Ia].
+//   k_slot_reduce: out[I] = alpha (rowacc[I] + sum_k slots[Ia][k][Ib])
+//       + beta cin[I], with the Taylor epilogue (FP64 sum, FP64 norm
+//       partials).  A streaming pass.
+//
+// The dvec never reaches HBM: per apply the traffic is C (cached), the
+// slot rows (n_alpha (norb - n_alpha) per determinant), rowacc and out.
+#include "fqehip_common.h"
+
+#include <cstdlib>
+
+namespace fqehip {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMPad = 128;  // output rows of the Y tile (npair + 1 <= 128)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __bf16 to_bf16(float x) {
+  return static_cast<__bf16>(x);  // round to nearest even (v_cvt_pk_bf16_f32)
+}
+
+__device__ __forceinline__ float from_bf16(__bf16 h) {
+  return __uint_as_float(static_cast<uint32_t>(__builtin_bit_cast(uint16_t, h))
+                         << 16);
+}
+
+__device__ __forceinline__ int32_t iabs(int32_t v) { return v < 0 ? -v : v; }
+
+// Raw buffer loads.  A buffer resource carries the byte size of the array;
+// the hardware returns 0 for an offset outside it, so absent table entries
+// are loaded with offset -1 instead of a branch around the load (all of a
+// thread's loads can then be in flight together).
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p,
+                                                            int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0,
+                                           static_cast<int>(bytes), 0x00020000);
+}
+
+__device__ __forceinline__ int32_t bload_i32(__amdgpu_buffer_rsrc_t r, int off) {
+  return static_cast<int32_t>(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+__device__ __forceinline__ float2 bload_f2(__amdgpu_buffer_rsrc_t r, int off) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
+}
+
+__device__ __forceinline__ double2 bload_d2(__amdgpu_buffer_rsrc_t r, int off) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+  const uint64_t lo = static_cast<uint64_t>(v[0]) | (static_cast<uint64_t>(v[1]) << 32);
+  const uint64_t hi = static_cast<uint64_t>(v[2]) | (static_cast<uint64_t>(v[3]) << 32);
+  return make_double2(__longlong_as_double(static_cast<long long>(lo)),
+                      __longlong_as_double(static_cast<long long>(hi)));
+}
+
+__device__ __forceinline__ float sgnf(int32_t v) {
+  return v > 0 ? 1.0f : (v < 0 ? -1.0f : 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// tables of the fused path
+// ---------------------------------------------------------------------------
+
+// fwd[p][src] = sign * (tgt + 1) where <tgt|E_p|src> = sign, from the gather
+// table; fwd is zero-filled first.
+__global__ void __launch_bounds__(kThreads)
+k_forward_table(int64_t total, int64_t len, const int32_t* __restrict__ pt,
+                int32_t* __restrict__ fwd) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (idx >= total) return;
+  const int32_t e = pt[idx];
+  if (!e) return;
+  const int64_t p = idx / len;
+  const int64_t tgt = idx - p * len;
+  const int64_t src = iabs(e) - 1;
+  fwd[p * len + src] = (e > 0 ? 1 : -1) * static_cast<int32_t>(tgt + 1);
+}
+
+// slot[p][src] = k for the off-diagonal de-excitation rows of every target,
+// numbered in dexc order; -1 elsewhere (pre-filled).
+__global__ void __launch_bounds__(kThreads)
+k_slot_table(int norb, int nexc, int64_t len, const int32_t* __restrict__ dexc,
+             int compressed, int32_t* __restrict__ slot) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= len) return;
+  int k = 0;
+  for (int e = 0; e < nexc; ++e) {
+    const int32_t* row = dexc + (t * nexc + e) * 3;
+    const int32_t src = row[0];
+    if (src == t) continue;
+    const int i = row[1] / norb;
+    const int j = row[1] % norb;
+    const int p = compressed ? comp_pair(i, j) : row[1];
+    slot[static_cast<int64_t>(p) * len + src] = k++;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_fill(int64_t n, int32_t v, int32_t* out) {
+  for (int64_t k = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+       k < n; k += static_cast<int64_t>(gridDim.x) * kThreads)
+    out[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Fused kernels.  The Y tile has 128 (padded) rows: one 16-row tile per
+// wave (8 waves).  Columns of the D / Y tiles are split into a real block
+// and an imaginary block, so the lane that holds Re Y[p][Ib] also holds
+// Im Y[p][Ib] and the scatter works on complex values.  Per alpha row the
+// workgroup caches in LDS: the alpha source offset and sign of every pair
+// (gather), and the slot-row base and sign of every pair's alpha target
+// (scatter); beta entries come from the pair / forward tables in L2.
+// ---------------------------------------------------------------------------
+constexpr int kFWaves = 8;
+constexpr int kFThreads = kFWaves * 64;
+constexpr int kRowTabs = 128;              // per-row LDS tables (>= kMPad)
+
+struct RowTables {
+  int32_t* aoff;    // [q] offset of C row src_a(q) (elements), -1 if none
+  int32_t* asgn;    // [q] sign of the alpha gather entry
+  int64_t* sbase;   // [p] slot row base of the alpha target, -1 none, -2 diag
+  int32_t* ssgn;    // [p] sign of the alpha target
+};
+
+template <typename T>
+__device__ __forceinline__ void load_row_tables(
+    int64_t ia, int K, int64_t lena, int64_t lenb, int nslot,
+    const int32_t* __restrict__ pta, const int32_t* __restrict__ fwa,
+    const int32_t* __restrict__ sla, RowTables rt, int tid, int nthreads) {
+  for (int q = tid; q < kRowTabs; q += nthreads) {
+    int32_t off = -1, sg = 0;
+    int64_t sb = -1;
+    int32_t ss = 0;
+    if (q < K) {
+      const int32_t a = pta[static_cast<int64_t>(q) * lena + ia];
+      if (a) {
+        off = static_cast<int32_t>((iabs(a) - 1) * lenb * 2);
+        sg = a > 0 ? 1 : -1;
+      }
+      const int32_t f = fwa[static_cast<int64_t>(q) * lena + ia];
+      if (f) {
+        const int64_t tgt = iabs(f) - 1;
+        ss = f > 0 ? 1 : -1;
+        sb = (tgt == ia) ? -2
+                         : (tgt * nslot + sla[static_cast<int64_t>(q) * lena + ia]) * lenb * 2;
+      }
+    }
+    rt.aoff[q] = off;
+    rt.asgn[q] = sg;
+    rt.sbase[q] = sb;
+    rt.ssgn[q] = ss;
+  }
+}
+
+// Scatter of one complex Y element (p, jb) of row ia (see file comment).
+template <typename T, typename A>
+__device__ __forceinline__ void scatter_complex(
+    A yr, A yi, int p, int npair, int64_t jb, int32_t fbv, RowTables rt,
+    A* accrow, T* __restrict__ slots) {
+  if (p == npair) {
+    atomicAdd(&accrow[2 * jb], yr);
+    atomicAdd(&accrow[2 * jb + 1], yi);
+    return;
+  }
+  if (fbv) {
+    const int64_t tb = 2 * (static_cast<int64_t>(iabs(fbv)) - 1);
+    atomicAdd(&accrow[tb], fbv > 0 ? yr : -yr);
+    atomicAdd(&accrow[tb + 1], fbv > 0 ? yi : -yi);
+  }
+  const int64_t sb = rt.sbase[p];
+  if (sb == -1) return;
+  const A s = static_cast<A>(rt.ssgn[p]);
+  if (sb == -2) {
+    atomicAdd(&accrow[2 * jb], s * yr);
+    atomicAdd(&accrow[2 * jb + 1], s * yi);
+  } else {
+    T* dst = slots + sb + 2 * jb;
+    dst[0] = static_cast<T>(s * yr);
+    dst[1] = static_cast<T>(s * yi);
+  }
+}
+
+// ---------------- mixed precision: bf16 hi/lo x3, FP32 accumulate ---------
+// 8 waves, <= 128 VGPRs: two workgroups (two alpha rows) per CU, so one
+// row's gather overlaps the other's MFMAs and scatter.
+constexpr int kBfTbC = 32;                 // complex columns per chunk
+constexpr int kBfKPad = 128;
+constexpr int kBfKSteps = kBfKPad / 32;
+constexpr int kBfStride = kBfKPad + 8;     // bf16 per LDS column (padding)
+constexpr int kBfCols = 2 * kBfTbC;        // re block then im block
+
+template <int NW, int WPE>
+__global__ void __launch_bounds__(NW * 64, WPE)
+k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
+               const double* __restrict__ amat,
+               const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+               const int32_t* __restrict__ fwa, const int32_t* __restrict__ fwb,
+               const int32_t* __restrict__ sla, int nslot,
+               const float* __restrict__ coeff, float* __restrict__ slots,
+               float* __restrict__ rowacc, const int32_t* __restrict__ done) {
+  constexpr int kT = NW * 64;              // threads
+  constexpr int kRT = 8 / NW;              // row tiles per wave
+  constexpr int kGroups = kT / kBfTbC;     // gather pair groups of 8
+  constexpr int kGIter = kBfKPad / (8 * kGroups);
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int M = npair + 1;
+  __bf16* dhi = reinterpret_cast<__bf16*>(smem);
+  __bf16* dlo = dhi + kBfCols * kBfStride;
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(dlo + kBfCols * kBfStride);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  float* accrow = reinterpret_cast<float*>(rt.ssgn + kRowTabs);
+
+  bf16x8 ahi[kRT][kBfKSteps], alo[kRT][kBfKSteps];
+#pragma unroll
+  for (int r = 0; r < kRT; ++r) {
+    const int row = (wave * kRT + r) * 16 + (lane & 15);
+#pragma unroll
+    for (int kt = 0; kt < kBfKSteps; ++kt) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = kt * 32 + 8 * (lane >> 4) + j;
+        const double v = (row < M && kk < K) ? amat[static_cast<int64_t>(row) * K + kk] : 0.0;
+        const __bf16 h = to_bf16(static_cast<float>(v));
+        ahi[r][kt][j] = h;
+        alo[r][kt][j] = to_bf16(static_cast<float>(v - static_cast<double>(from_bf16(h))));
+      }
+    }
+  }
+  const bool wave_live = wave * kRT * 16 <= npair;
+  const int t = tid % kBfTbC;            // gather column
+  const int g = tid / kBfTbC;            // gather pair group
+  const int nchunks = static_cast<int>((lenb + kBfTbC - 1) / kBfTbC);
+  const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(float));
+  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+  const __amdgpu_buffer_rsrc_t rfb = buffer_of(fwb, static_cast<int64_t>(K) * lenb * 4);
+
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+    load_row_tables<float>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                           kT);
+    for (int64_t x = tid; x < 2 * lenb; x += kT) accrow[x] = 0.0f;
+    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(float));
+    int32_t bt[kGIter][8];                 // gather-table entries, this chunk
+    auto load_bt = [&](int64_t ib0) {
+      const int ib = static_cast<int>(ib0) + t;
+#pragma unroll
+      for (int it = 0; it < kGIter; ++it)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int q = 8 * (g + it * kGroups) + j;
+          bt[it][j] = bload_i32(rpb, (ib < lenb && q < K)
+                                         ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+        }
+    };
+    load_bt(0);
+    __syncthreads();
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
+      const int64_t ib0 = static_cast<int64_t>(chunk) * kBfTbC;
+      {
+        const int ib = static_cast<int>(ib0) + t;
+        const bool colok = ib < lenb;
+#pragma unroll
+        for (int it = 0; it < kGIter; ++it) {
+          const int q0 = 8 * (g + it * kGroups);
+          bf16x8 hr, lr, hi, li;
+          float2 ca[8], cb[8];
+          float sa[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int q = q0 + j;
+            const int32_t off = (colok && q < K) ? rt.aoff[q] : -1;
+            sa[j] = off >= 0 ? static_cast<float>(rt.asgn[q]) : 0.0f;
+            ca[j] = bload_f2(rc, off >= 0 ? (off + 2 * ib) * 4 : -1);
+            const int32_t b = bt[it][j];
+            cb[j] = bload_f2(rc, b ? crow_b + 8 * (iabs(b) - 1) : -1);
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const float sb = sgnf(bt[it][j]);
+            const float re = sa[j] * ca[j].x + sb * cb[j].x;
+            const float im = sa[j] * ca[j].y + sb * cb[j].y;
+            const __bf16 h0 = to_bf16(re), h1 = to_bf16(im);
+            hr[j] = h0;
+            lr[j] = to_bf16(re - from_bf16(h0));
+            hi[j] = h1;
+            li[j] = to_bf16(im - from_bf16(h1));
+          }
+          *reinterpret_cast<bf16x8*>(dhi + t * kBfStride + q0) = hr;
+          *reinterpret_cast<bf16x8*>(dlo + t * kBfStride + q0) = lr;
+          *reinterpret_cast<bf16x8*>(dhi + (kBfTbC + t) * kBfStride + q0) = hi;
+          *reinterpret_cast<bf16x8*>(dlo + (kBfTbC + t) * kBfStride + q0) = li;
+        }
+      }
+      __syncthreads();
+      // the next chunk's gather table is in flight during the MFMAs
+      if (chunk + 1 < nchunks) load_bt(ib0 + kBfTbC);
+      if (wave_live) {
+        // two passes over 16-column halves h: tile h holds Re, tile h+2 Im
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          int32_t fbv[kRT][4];
+#pragma unroll
+          for (int r = 0; r < kRT; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
+              const int jb = static_cast<int>(ib0) + h * 16 + (lane & 15);
+              fbv[r][i] = bload_i32(rfb, (p < npair && jb < lenb)
+                                             ? (p * static_cast<int>(lenb) + jb) * 4 : -1);
+            }
+          floatx4 are[kRT], aim[kRT];
+#pragma unroll
+          for (int r = 0; r < kRT; ++r) {
+            are[r] = floatx4{0, 0, 0, 0};
+            aim[r] = floatx4{0, 0, 0, 0};
+          }
+#pragma unroll
+          for (int kt = 0; kt < kBfKSteps; ++kt) {
+            const int kq = kt * 32 + 8 * (lane >> 4);
+            const int cre = h * 16 + (lane & 15);
+            const int cim = kBfTbC + cre;
+            const bf16x8 bhr = *reinterpret_cast<const bf16x8*>(dhi + cre * kBfStride + kq);
+            const bf16x8 blr = *reinterpret_cast<const bf16x8*>(dlo + cre * kBfStride + kq);
+            const bf16x8 bhi = *reinterpret_cast<const bf16x8*>(dhi + cim * kBfStride + kq);
+            const bf16x8 bli = *reinterpret_cast<const bf16x8*>(dlo + cim * kBfStride + kq);
+#pragma unroll
+            for (int r = 0; r < kRT; ++r) {
+              are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhr, are[r], 0, 0, 0);
+              aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhi, aim[r], 0, 0, 0);
+              are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], blr, are[r], 0, 0, 0);
+              aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bli, aim[r], 0, 0, 0);
+              are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhr, are[r], 0, 0, 0);
+              aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhi, aim[r], 0, 0, 0);
+            }
+          }
+          // f32 MFMA layout: item i of lane l is row 4*(l/16)+i, col l%16
+#pragma unroll
+          for (int r = 0; r < kRT; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
+              const int64_t jb = ib0 + h * 16 + (lane & 15);
+              if (p <= npair && jb < lenb)
+                scatter_complex<float, float>(are[r][i], aim[r][i], p, npair,
+                                              jb, fbv[r][i], rt, accrow, slots);
+            }
+        }
+      }
+      __syncthreads();
+    }
+    float* dst = rowacc + ia * 2 * lenb;
+    for (int64_t x = tid; x < 2 * lenb; x += kT) dst[x] = accrow[x];
+    __syncthreads();
+  }
+}
+
+// ---------------- FP64: v_mfma_f64_16x16x4f64 -----------------------------
+// One workgroup per CU (the FP64 row accumulator fills the LDS); the D tile
+// is double-buffered so the gather of chunk c+1 (coefficients loaded one
+// chunk ahead, table entries two chunks ahead) overlaps the FP64 MFMAs of
+// chunk c, with one barrier per chunk.
+constexpr int kF64TbC = 16;                // complex columns per chunk
+constexpr int kF64Cols = 2 * kF64TbC;      // re block then im block
+constexpr int kF64KPad = 128;              // gather rows (MFMA uses ceil(K/4))
+constexpr int kF64KMax = 112;              // npair limit of the FP64 kernel
+constexpr int kF64KSteps = kF64KMax / 4;
+constexpr int kF64Stride = kF64Cols + 1;   // doubles per LDS row (padding)
+constexpr int kF64Groups = kFThreads / kF64TbC;
+constexpr int kF64GIter = kF64KPad / kF64Groups;
+constexpr int kF64Tile = kF64KPad * kF64Stride;
+
+__global__ void __launch_bounds__(kFThreads, 2)
+k_fused_f64(int npair, int64_t lena, int64_t lenb,
+            const double* __restrict__ amat, const int32_t* __restrict__ pta,
+            const int32_t* __restrict__ ptb, const int32_t* __restrict__ fwa,
+            const int32_t* __restrict__ fwb, const int32_t* __restrict__ sla,
+            int nslot, const double* __restrict__ coeff,
+            double* __restrict__ slots, double* __restrict__ rowacc,
+            const int32_t* __restrict__ done) {
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int M = npair + 1;
+  const int ksteps = (K + 3) / 4;
+  double* dt = reinterpret_cast<double*>(smem);     // 2 x [kF64KPad][kF64Stride]
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(dt + 2 * kF64Tile);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  double* accrow = reinterpret_cast<double*>(rt.ssgn + kRowTabs);
+
+  double areg[kF64KSteps];
+  {
+    const int row = wave * 16 + (lane & 15);
+#pragma unroll
+    for (int kt = 0; kt < kF64KSteps; ++kt) {
+      const int kk = kt * 4 + (lane >> 4);
+      areg[kt] = (row < M && kk < K) ? amat[static_cast<int64_t>(row) * K + kk] : 0.0;
+    }
+  }
+  const bool wave_live = wave * 16 <= npair;
+  const int t = tid % kF64TbC;
+  const int g = tid / kF64TbC;
+  const int nchunks = static_cast<int>((lenb + kF64TbC - 1) / kF64TbC);
+  const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(double));
+  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+  const __amdgpu_buffer_rsrc_t rfb = buffer_of(fwb, static_cast<int64_t>(K) * lenb * 4);
+
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+    load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                            kFThreads);
+    for (int64_t x = tid; x < 2 * lenb; x += kFThreads) accrow[x] = 0.0;
+    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(double));
+    __syncthreads();
+
+    int32_t bt[kF64GIter];                 // gather-table entries (ahead)
+    double2 ca[kF64GIter], cb[kF64GIter];  // coefficients (one chunk ahead)
+    int32_t as[kF64GIter], bs[kF64GIter];
+    auto load_bt = [&](int64_t ib0) {
+      const int ib = static_cast<int>(ib0) + t;
+#pragma unroll
+      for (int it = 0; it < kF64GIter; ++it) {
+        const int q = g + it * kF64Groups;
+        bt[it] = bload_i32(rpb, (ib < lenb && q < K)
+                                    ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+      }
+    };
+    auto load_coeff = [&](int64_t ib0) {
+      const int ib = static_cast<int>(ib0) + t;
+      const bool colok = ib < lenb;
+#pragma unroll
+      for (int it = 0; it < kF64GIter; ++it) {
+        const int q = g + it * kF64Groups;
+        const int32_t off = (colok && q < K) ? rt.aoff[q] : -1;
+        as[it] = off >= 0 ? rt.asgn[q] : 0;
+        ca[it] = bload_d2(rc, off >= 0 ? (off + 2 * ib) * 8 : -1);
+        bs[it] = bt[it];
+        cb[it] = bload_d2(rc, bt[it] ? crow_b + 16 * (iabs(bt[it]) - 1) : -1);
+      }
+    };
+    auto store_tile = [&](double* tile) {
+#pragma unroll
+      for (int it = 0; it < kF64GIter; ++it) {
+        const int q = g + it * kF64Groups;
+        const double sa = static_cast<double>(as[it]);
+        const double sb = bs[it] > 0 ? 1.0 : (bs[it] < 0 ? -1.0 : 0.0);
+        tile[q * kF64Stride + t] = sa * ca[it].x + sb * cb[it].x;
+        tile[q * kF64Stride + kF64TbC + t] = sa * ca[it].y + sb * cb[it].y;
+      }
+    };
+    // prologue: tile 0 in buffer 0, tables of chunk 1 in registers
+    load_bt(0);
+    load_coeff(0);
+    if (nchunks > 1) load_bt(kF64TbC);
+    store_tile(dt);
+    __syncthreads();
+
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
+      const int64_t ib0 = static_cast<int64_t>(chunk) * kF64TbC;
+      const double* cur = dt + (chunk & 1) * kF64Tile;
+      double* nxt = dt + ((chunk + 1) & 1) * kF64Tile;
+      const bool more = chunk + 1 < nchunks;
+      if (more) load_coeff(ib0 + kF64TbC);
+      if (chunk + 2 < nchunks) load_bt(ib0 + 2 * kF64TbC);
+      int32_t fbv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = wave * 16 + (lane >> 4) + 4 * i;
+        const int jb = static_cast<int>(ib0) + (lane & 15);
+        fbv[i] = bload_i32(rfb, (wave_live && p < npair && jb < lenb)
+                                    ? (p * static_cast<int>(lenb) + jb) * 4 : -1);
+      }
+      double4_t acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
+      if (wave_live) {
+#pragma unroll
+        for (int kt = 0; kt < kF64KSteps; ++kt) {
+          if (kt < ksteps) {
+            const double* r = cur + (kt * 4 + (lane >> 4)) * kF64Stride + (lane & 15);
+            acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[kt], r[0], acc_re, 0, 0, 0);
+            acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[kt], r[kF64TbC], acc_im, 0, 0, 0);
+          }
+        }
+      }
+      if (more) store_tile(nxt);
+      if (wave_live) {
+        // f64 MFMA layout: item i of lane l is row l/16 + 4 i, col l%16
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int p = wave * 16 + (lane >> 4) + 4 * i;
+          const int64_t jb = ib0 + (lane & 15);
+          if (p <= npair && jb < lenb)
+            scatter_complex<double, double>(acc_re[i], acc_im[i], p, npair, jb,
+                                            fbv[i], rt, accrow, slots);
+        }
+      }
+      __syncthreads();
+    }
+    double* dst = rowacc + ia * 2 * lenb;
+    for (int64_t x = tid; x < 2 * lenb; x += kFThreads) dst[x] = accrow[x];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// slot reduction + epilogue
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
+              const T* __restrict__ slots, const T* __restrict__ rowacc,
+              double alpha_re, double alpha_im, double beta_re, double beta_im,
+              const T* cin, T* out, double* __restrict__ acc,
+              double* __restrict__ partials, const int32_t* __restrict__ done) {
+  __shared__ double red[kWaves];
+  if (done && *done) return;
+  const int64_t n = lena * lenb;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  double sq = 0.0;
+  if (idx < n) {
+    int64_t ia = static_cast<int64_t>(static_cast<double>(idx) * inv_lenb);
+    int64_t ib = idx - ia * lenb;
+    if (ib < 0) { --ia; ib += lenb; } else if (ib >= lenb) { ++ia; ib -= lenb; }
+    double re = static_cast<double>(rowacc[2 * idx]);
+    double im = static_cast<double>(rowacc[2 * idx + 1]);
+    const T* s = slots + (ia * nslot * lenb + ib) * 2;
+    for (int k = 0; k < nslot; ++k) {
+      re += static_cast<double>(s[0]);
+      im += static_cast<double>(s[1]);
+      s += 2 * lenb;
+    }
+    double vr = alpha_re * re - alpha_im * im;
+    double vi = alpha_re * im + alpha_im * re;
+    if (beta_re != 0.0 || beta_im != 0.0) {
+      const double cr = static_cast<double>(cin[2 * idx]);
+      const double ci = static_cast<double>(cin[2 * idx + 1]);
+      vr += beta_re * cr - beta_im * ci;
+      vi += beta_re * ci + beta_im * cr;
+    }
+    const T orr = static_cast<T>(vr);
+    const T oi = static_cast<T>(vi);
+    out[2 * idx] = orr;
+    out[2 * idx + 1] = oi;
+    if (acc) {
+      acc[2 * idx] += vr;
+      acc[2 * idx + 1] += vi;
+    }
+    sq = static_cast<double>(orr) * orr + static_cast<double>(oi) * oi;
+  }
+  if (partials) {
+    for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int w = 0; w < kWaves; ++w) t += red[w];
+      partials[blockIdx.x] = t;
+    }
+  }
+}
+
+size_t row_tables_bytes() {
+  return kRowTabs * (sizeof(int64_t) + 3 * sizeof(int32_t));
+}
+
+size_t bf16_lds_bytes(int64_t lenb) {
+  return 2 * kBfCols * kBfStride * sizeof(__bf16) + row_tables_bytes() +
+         2 * lenb * sizeof(float);
+}
+
+size_t f64_lds_bytes(int64_t lenb) {
+  return 2 * kF64Tile * sizeof(double) + row_tables_bytes() +
+         2 * lenb * sizeof(double);
+}
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+// Tile variant of the mixed-precision fused kernel: FQEHIP_BF16_WAVES=8
+// selects 8 waves per workgroup, anything else 4.
+int bf16_variant() {
+  static const int v = [] {
+    const char* e = getenv("FQEHIP_BF16_WAVES");
+    return (e && atoi(e) == 8) ? 8 : 4;
+  }();
+  return v;
+}
+
+unsigned fused_grid(int64_t lena) {
+  return static_cast<unsigned>(lena < 65536 ? lena : 65536);
+}
+
+}  // namespace
+
+// Whether the fused path can run this sector / operator.
+bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb) {
+  if (npair + 1 > kMPad) return false;
+  // byte offsets of C and of the pair tables are 32-bit (buffer loads)
+  const int64_t lim = int64_t(1) << 31;
+  if (static_cast<int64_t>(npair) * lenb * 4 >= lim) return false;
+  if (precision == FQEHIP_C128)
+    return npair <= kF64KMax && f64_lds_bytes(lenb) <= kLdsLimit &&
+           lena * lenb * 2 * 8 < lim;
+  return npair <= kBfKPad && bf16_lds_bytes(lenb) <= kLdsLimit &&
+         lena * lenb * 2 * 4 < lim;
+}
+
+int launch_fused(int precision, int npair, int64_t lena, int64_t lenb,
+                 const double* a, const int32_t* pta, const int32_t* ptb,
+                 const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
+                 int nslot, const void* coeff, void* slots, void* rowacc,
+                 hipStream_t st, const int32_t* done) {
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  if (precision == FQEHIP_C128) {
+    const size_t lds = f64_lds_bytes(lenb);
+    k_fused_f64<<<fused_grid(lena), kFThreads, lds, st>>>(
+        npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
+        static_cast<const double*>(coeff), static_cast<double*>(slots),
+        static_cast<double*>(rowacc), done);
+    FQEHIP_CHECK_LAUNCH("k_fused_f64");
+  } else {
+    const size_t lds = bf16_lds_bytes(lenb);
+    // variant 4: 4 waves x 2 row tiles, two workgroups per CU;
+    // variant 8: 8 waves x 1 row tile, one workgroup per CU
+    if (bf16_variant() == 8)
+      k_fused_bf16x3<8, 2><<<fused_grid(lena), 512, lds, st>>>(
+          npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
+          static_cast<const float*>(coeff), static_cast<float*>(slots),
+          static_cast<float*>(rowacc), done);
+    else
+      k_fused_bf16x3<4, 2><<<fused_grid(lena), 256, lds, st>>>(
+          npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
+          static_cast<const float*>(coeff), static_cast<float*>(slots),
+          static_cast<float*>(rowacc), done);
+    FQEHIP_CHECK_LAUNCH("k_fused_bf16x3");
+  }
+  return FQEHIP_OK;
+}
+
+int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
+                       const void* slots, const void* rowacc, double ar,
+                       double ai, double br, double bi, const void* cin,
+                       void* out, double* acc, double* partials,
+                       hipStream_t st, const int32_t* done) {
+  const int64_t n = lena * lenb;
+  if (n == 0) return FQEHIP_OK;
+  const unsigned nb = blocks_for(n, kThreads);
+  if (precision == FQEHIP_C128) {
+    k_slot_reduce<double><<<nb, kThreads, 0, st>>>(
+        nslot, lena, lenb, 1.0 / lenb, static_cast<const double*>(slots),
+        static_cast<const double*>(rowacc), ar, ai, br, bi,
+        static_cast<const double*>(cin), static_cast<double*>(out), acc,
+        partials, done);
+  } else {
+    k_slot_reduce<float><<<nb, kThreads, 0, st>>>(
+        nslot, lena, lenb, 1.0 / lenb, static_cast<const float*>(slots),
+        static_cast<const float*>(rowacc), ar, ai, br, bi,
+        static_cast<const float*>(cin), static_cast<float*>(out), acc,
+        partials, done);
+  }
+  FQEHIP_CHECK_LAUNCH("k_slot_reduce");
+  return FQEHIP_OK;
+}
+
+}  // namespace fqehip
+
+using namespace fqehip;
+
+extern "C" int fqehip_fused_tables(int norb, int nele, int64_t nstrings,
+                                   const int32_t* d_dexc, int compressed,
+                                   const int32_t* d_ptab, int32_t* d_fwd,
+                                   int32_t* d_slot, void* stream) {
+  clear_error();
+  FQEHIP_CHECK_ARG(norb >= 0 && norb <= kMaxNorb && nele >= 0 && nele <= norb,
+                   "bad sector norb=%d nele=%d", norb, nele);
+  FQEHIP_CHECK_ARG(compressed == 0 || compressed == 1, "compressed=%d",
+                   compressed);
+  FQEHIP_CHECK_ARG(d_ptab && d_fwd, "NULL table pointer");
+  const int npair = compressed ? norb * (norb + 1) / 2 : norb * norb;
+  const int64_t total = static_cast<int64_t>(npair) * nstrings;
+  hipStream_t st = as_stream(stream);
+  if (total == 0) return FQEHIP_OK;
+  const unsigned nb = blocks_for(total, kThreads) < 4096u ? blocks_for(total, kThreads) : 4096u;
+  k_fill<<<nb, kThreads, 0, st>>>(total, 0, d_fwd);
+  FQEHIP_CHECK_LAUNCH("k_fill");
+  k_forward_table<<<blocks_for(total, kThreads), kThreads, 0, st>>>(
+      total, nstrings, d_ptab, d_fwd);
+  FQEHIP_CHECK_LAUNCH("k_forward_table");
+  if (d_slot) {
+    k_fill<<<nb, kThreads, 0, st>>>(total, -1, d_slot);
+    FQEHIP_CHECK_LAUNCH("k_fill");
+    const int nexc = nele * (norb - nele + 1);
+    if (nexc > 0) {
+      FQEHIP_CHECK_ARG(d_dexc != nullptr, "d_dexc is NULL");
+      k_slot_table<<<blocks_for(nstrings, kThreads), kThreads, 0, st>>>(
+          norb, nexc, nstrings, d_dexc, compressed, d_slot);
+      FQEHIP_CHECK_LAUNCH("k_slot_table");
+    }
+  }
+  return FQEHIP_OK;
+}

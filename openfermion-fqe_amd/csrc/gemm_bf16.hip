@@ -17,10 +17,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // Round-to-nearest-even float -> bfloat16 (finite inputs).
 __device__ __forceinline__ __bf16 to_bf16(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  const uint16_t h = static_cast<uint16_t>(u >> 16);
-  return __builtin_bit_cast(__bf16, h);
+  return static_cast<__bf16>(x);  // round to nearest even (v_cvt_pk_bf16_f32)
 }
 
 __device__ __forceinline__ float from_bf16(__bf16 h) {

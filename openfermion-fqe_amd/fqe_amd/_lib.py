@@ -35,7 +35,9 @@ class FqeHipError(RuntimeError):
 class Sector(ctypes.Structure):
     _fields_ = [("norb", c_int), ("nalpha", c_int), ("nbeta", c_int),
                 ("npair", c_int), ("lena", c_int64), ("lenb", c_int64),
-                ("ptab_a", c_void_p), ("ptab_b", c_void_p)]
+                ("ptab_a", c_void_p), ("ptab_b", c_void_p),
+                ("fwd_a", c_void_p), ("fwd_b", c_void_p),
+                ("slot_a", c_void_p), ("nslot", c_int)]
 
 
 class Operator(ctypes.Structure):
@@ -71,6 +73,18 @@ _SIGNATURES = {
                                          c_void_p]),
     "fqehip_convert_to_c128": (c_int, [c_int, c_int64, c_void_p, c_double,
                                        c_double, c_void_p, c_void_p]),
+    "fqehip_fused_tables": (c_int, [c_int, c_int, c_int64, c_void_p, c_int,
+                                    c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fqehip_fused_supported": (c_int, [c_int, POINTER(Sector),
+                                       POINTER(Operator)]),
+    "fqehip_apply_fused_rows": (c_int, [c_int, POINTER(Sector),
+                                        POINTER(Operator), c_void_p, c_void_p,
+                                        c_void_p, c_void_p]),
+    "fqehip_apply_slot_reduce": (c_int, [c_int, POINTER(Sector), c_void_p,
+                                         c_double, c_double, c_double,
+                                         c_double, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p,
+                                         c_void_p]),
     "fqehip_apply_workspace_bytes": (c_int64, [c_int, POINTER(Sector)]),
     "fqehip_apply": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
                              c_void_p, c_double, c_double, c_double, c_double,
@@ -119,7 +133,7 @@ def ptr(t) -> int:
 
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
-    if name.endswith("_count") or name.endswith("_bytes"):
+    if name.endswith(("_count", "_bytes", "_supported")):
         return rc
     if rc != OK:
         msg = lib().fqehip_last_error().decode(errors="replace")

@@ -23,6 +23,7 @@ import numpy
 import torch
 
 from . import _lib
+from . import settings
 
 
 class PreparedOperator:
@@ -116,9 +117,17 @@ def sector_struct(graph, op: PreparedOperator) -> Tuple[_lib.Sector, tuple]:
     element keeps the tables referenced."""
     ta = graph.tables(True).pair_table(op.compressed)
     tb = graph.tables(False).pair_table(op.compressed)
+    keep = [ta, tb]
+    fwa = fwb = sla = None
+    if settings.fused and not op.complex_op and op.h1 is None:
+        fwa, sla = graph.tables(True).fused_tables(op.compressed)
+        fwb, _ = graph.tables(False).fused_tables(op.compressed)
+        keep += [fwa, fwb, sla]
+    nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
     s = _lib.Sector(graph.norb(), graph.nalpha(), graph.nbeta(), op.npair,
-                    graph.lena(), graph.lenb(), _lib.ptr(ta), _lib.ptr(tb))
-    return s, (ta, tb)
+                    graph.lena(), graph.lenb(), _lib.ptr(ta), _lib.ptr(tb),
+                    _lib.ptr(fwa), _lib.ptr(fwb), _lib.ptr(sla), nslot)
+    return s, tuple(keep)
 
 
 def workspace(sector: _lib.Sector, precision: int,
@@ -128,7 +137,8 @@ def workspace(sector: _lib.Sector, precision: int,
     nbytes = int(_lib.lib().fqehip_apply_workspace_bytes(precision,
                                                           _as_ref(sector)))
     elem = 8 if precision == _lib.C128 else 4
-    need = (2 * sector.npair + 1) * 2 * sector.lena * sector.lenb * elem
+    n = sector.lena * sector.lenb
+    need = max(2 * sector.npair + 1, sector.nslot + 1) * 2 * n * elem
     if nbytes != need:
         raise _lib.FqeHipError(f"workspace size {nbytes} != {need}")
     nbytes = max(nbytes, 16 * sector.lena * sector.lenb, 16)

@@ -58,6 +58,23 @@ class SpinTables:
             self._ptab[key] = tab
         return self._ptab[key]
 
+    def fused_tables(self, compressed: bool):
+        """(forward table, slot table) of the fused apply, int32
+        [npair][length] each (see fqehip_fused_tables)."""
+        key = ("fused", bool(compressed))
+        if key not in self._ptab:
+            pt = self.pair_table(compressed)
+            fwd = torch.empty_like(pt)
+            slot = torch.empty_like(pt)
+            if pt.numel():
+                _lib.call("fqehip_fused_tables", self.norb, self.nele,
+                          self.length,
+                          _lib.ptr(self.dexc) if self.dexc.numel() else None,
+                          int(bool(compressed)), _lib.ptr(pt), _lib.ptr(fwd),
+                          _lib.ptr(slot), _lib.stream_ptr())
+            self._ptab[key] = (fwd, slot)
+        return self._ptab[key]
+
     def map_lengths(self) -> numpy.ndarray:
         n, k = self.norb, self.nele
         off = comb(n - 2, k - 1) if n >= 2 and k >= 1 else 0

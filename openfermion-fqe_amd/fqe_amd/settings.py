@@ -13,3 +13,7 @@ default_dtype = numpy.complex128
 # Maximum number of orbitals of the GPU string tables (uint64 strings,
 # int32-addressed tables).
 max_norb = 32
+
+# Use the fused gather + GEMM + scatter kernels for real operators when the
+# sector fits them (else, and for complex operators, the unfused kernels).
+fused = True

@@ -343,3 +343,58 @@ def test_mixed_precision_apply_and_evolve(fqe, na, nb, norb):
     err = rel(wfn.time_evolve(0.1, ham).get_coeff(key), ref)
     print(f"mixed evolve rel err {err:.3e}", flush=True)
     assert err < MIXED_TOL
+
+
+# --------------------------------------------------------------------------
+# fused (gather + GEMM + slot scatter) and unfused kernels
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("fused", [True, False])
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+@pytest.mark.parametrize("na,nb,norb,kind", [(3, 0, 7, "symmetric"),
+                                             (5, 5, 10, "symmetric"),
+                                             (4, 4, 9, "real"),
+                                             (7, 3, 10, "symmetric"),
+                                             (2, 6, 8, "real"),
+                                             (1, 1, 3, "real")])
+def test_fused_and_unfused_paths(fqe, fused, dtype, na, nb, norb, kind):
+    import ctypes
+    from fqe_amd import settings, engine, _lib
+    print(f"paths fused={fused} {dtype} {na} {nb} {norb} {kind}", flush=True)
+    rng = numpy.random.default_rng(na * 100 + nb * 10 + norb)
+    h1, h2 = _h(norb, rng, kind)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    ref = fo.apply_12(c, h1, h2, g)
+    npdt = numpy.complex128 if dtype == "c128" else numpy.complex64
+    saved = settings.fused
+    settings.fused = fused
+    try:
+        d = fqe.FqeData(na, nb, norb, dtype=npdt)
+        d.coeff = c
+        op = engine.prepare_operator((h1, h2), norb, _lib.device())
+        s, keep = engine.sector_struct(d.get_fcigraph(), op)
+        prec = engine.precision_of(d.device_coeff().dtype)
+        took = _lib.lib().fqehip_fused_supported(prec, ctypes.byref(s),
+                                                 ctypes.byref(op.c_struct()))
+        assert bool(took) == fused
+        out = d.apply((h1, h2)).coeff
+    finally:
+        settings.fused = saved
+    tol = FP64_TOL if dtype == "c128" else MIXED_TOL
+    assert rel(out, ref) < tol
+
+
+def test_fused_taylor_with_e0(fqe):
+    """Time evolution through the fused kernels (real symmetric h)."""
+    norb, na, nb = 9, 4, 3
+    c, h1, h2 = fo.synthetic_workload(norb, na, nb, 7, norb)
+    g = fo.OracleGraph(na, nb, norb)
+    ref, nterms = fo.taylor_evolve(c, (h1, h2), 0.3 + 0.1j, 0.05, g)
+    key = (na + nb, na - nb)
+    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]])
+    wfn.set_wfn(strategy="from_data", raw_data={key: c})
+    out = wfn.time_evolve(0.05, fqe.get_restricted_hamiltonian((h1, h2),
+                                                               e_0=0.3 + 0.1j))
+    assert out._taylor_terms == nterms
+    assert rel(out.get_coeff(key), ref) < FP64_TOL
