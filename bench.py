@@ -72,10 +72,10 @@ class EngineRun:
         dev = _lib.device()
         self.c = torch.from_numpy(coeff_np).to(dev).to(torch_dtype)
         self.out = torch.empty_like(self.c)
-        self.prec = engine.precision_of(torch_dtype)
         saved = settings.fused
         settings.fused = fused
-        self.s, self.keep = engine.sector_struct(graph, op)
+        self.prec = engine.precision_of(torch_dtype)
+        self.s, self.keep = engine.sector_struct(graph, op, self.prec)
         settings.fused = saved
         self.opc = op.c_struct()
         self.sref = ctypes.byref(self.s)
@@ -86,6 +86,8 @@ class EngineRun:
         self.n = self.c.numel()
         esz = 8 if self.prec == _lib.C128 else 4
         self.y_ptr = self.ws.data_ptr() + op.npair * 2 * self.n * esz
+        self.split = bool(self.s.hbb_hi)
+        self.rowacc_ptr = self.ws.data_ptr() + self.s.nslot * 2 * self.n * esz
         self.names = (["fused_rows", "slot_reduce"] if self.fused else
                       ["dvec_gather", "pair_gemm", "scatter"])
 
@@ -140,6 +142,21 @@ class EngineRun:
         wall = time.perf_counter() - t0
         per = {nm: sum(e[i].elapsed_time(e[i + 1]) for e in evs) / steps
                for i, nm in enumerate(self.names)}
+        if self.split:
+            # fused_rows = beta-beta GEMM + alpha row kernel; the GEMM is
+            # timed alone (same stream, same buffers) to split the two
+            L, s = self._lib, self.s
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
+            for _ in range(steps):
+                L.call("fqehip_beta_gemm", s.lena, s.lenb, s.hbb_hi, s.hbb_lo,
+                       s.ldh, L.ptr(self.c), self.rowacc_ptr, None,
+                       L.stream_ptr())
+            ev[1].record()
+            torch.cuda.synchronize()
+            tb = ev[0].elapsed_time(ev[1]) / steps
+            rows = per.pop("fused_rows")
+            per = {"beta_gemm": tb, "alpha_rows": rows - tb, **per}
         return wall, per
 
     def result(self):
@@ -153,11 +170,20 @@ class EngineRun:
 def kernel_work(op, graph, n, esz, mixed):
     """Algorithmic bytes and flops of one launch of each kernel.  GEMM
     flops are those of the real (npair+1) x npair pair matrix on 2n real
-    columns; the mixed engine executes three BF16 products per flop."""
+    columns; the mixed engine executes three BF16 products per flop.
+    Split apply: the alpha row kernel multiplies (npair+1) x (npair+1)
+    (extra h1' column), the beta-beta GEMM is lena x lenb x lenb complex
+    times real."""
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
     flops = 2.0 * (p + 1) * p * 2 * n
+    lenb = graph.lenb()
+    ldh = ((lenb + 127) // 128) * 128
     return {
+        "beta_gemm": {"bytes": 2 * 2 * n * esz + 2 * ldh * ldh * 2,
+                      "flops": 2.0 * 2 * n * lenb},
+        "alpha_rows": {"bytes": (nslot + 3) * 2 * n * esz,
+                       "flops": 2.0 * (p + 1) * (p + 1) * 2 * n},
         "dvec_gather": {"bytes": (p + 1) * 2 * n * esz},
         "pair_gemm": {"bytes": (2 * p + 1) * 2 * n * esz, "flops": flops},
         "scatter": {"bytes": (p + 1 + 1) * 2 * n * esz},

@@ -167,6 +167,13 @@ typedef struct fqehip_sector {
   const int32_t* fwd_b;    /* [npair][lenb] forward table, beta */
   const int32_t* slot_a;   /* [npair][lena] alpha slot index */
   int nslot;               /* nalpha * (norb - nalpha) */
+  /* Beta-beta block of the operator on this sector (fqehip_beta_block),
+   * bfloat16 bit patterns [ldh][ldh]; set only for an operator with
+   * A[p][q] = A[q][p] (p, q < npair).  When set, complex64 applies take
+   * the split path; NULL keeps the beta excitations in the row kernel. */
+  const uint16_t* hbb_hi;
+  const uint16_t* hbb_lo;
+  int64_t ldh;
 } fqehip_sector;
 
 typedef struct fqehip_operator {
@@ -195,6 +202,28 @@ int fqehip_fused_tables(int norb, int nele, int64_t nstrings,
  *   fqehip_scatter. */
 int fqehip_fused_supported(int precision, const fqehip_sector* s,
                            const fqehip_operator* op);
+
+/* Beta-beta block of a pair-symmetric real operator (A as in
+ * fqehip_operator, A[p][q] = A[q][p] for p, q < npair):
+ *   Hbb = sum_pq A[p][q] E^b_p E^b_q + sum_q A[npair][q] E^b_q
+ * on the lenb beta strings, from the beta forward table d_fwd_b.  Writes
+ * d_hcol[Ib][Ib'] = Hbb[Ib'][Ib] (FP64, lenb x lenb, scratch the caller
+ * may free afterwards) and the bfloat16 hi + lo split of Hbb, row-major
+ * [ldh][ldh] zero-padded, ldh = fqehip_beta_block_ld(lenb).  It is the
+ * same-spin part of the operator that the reference's lm kernels apply
+ * per spin (src/fqe/lib/fqe_data.c:668-720); here it is a matrix so that
+ * the beta half of every apply is one GEMM.  lenb <= 20480. */
+int64_t fqehip_beta_block_ld(int64_t lenb);
+int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b,
+                      const double* d_a, double* d_hcol, uint16_t* d_hhi,
+                      uint16_t* d_hlo, int64_t ldh, void* stream);
+/* d_out[Ia][Ib'] = sum_Ib d_coeff[Ia][Ib] Hbb[Ib'][Ib] for complex64
+ * d_coeff / d_out ([lena][lenb] interleaved), BF16x3 products with FP32
+ * accumulation.  The opposite-spin and alpha parts of the split apply
+ * run in fqehip_apply_fused_rows after it. */
+int fqehip_beta_gemm(int64_t lena, int64_t lenb, const uint16_t* d_hhi,
+                     const uint16_t* d_hlo, int64_t ldh, const void* d_coeff,
+                     void* d_out, const int32_t* d_done, void* stream);
 int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
                             const fqehip_operator* op, const void* d_coeff,
                             void* d_workspace, const int32_t* d_done,
@@ -238,6 +267,85 @@ int fqehip_taylor(int precision, const fqehip_sector* s,
                   int expansion, void* d_term, double* d_result,
                   int32_t* d_status, double* d_partials, double* d_norm,
                   void* d_workspace, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Davidson eigensolver kernels (the reference's davidsonliu_fqe,
+ * src/fqe/algorithm/davidson.py:140-307, builds its preconditioner from
+ * one expectation value per determinant, :160-180; here the diagonal is
+ * the Slater-Condon closed form).  Vectors: complex128 [n] interleaved; a
+ * block of k vectors is k rows of stride ldv elements.
+ * ------------------------------------------------------------------- */
+/* d_diag[Ia][Ib] = <I|H|I> = sum_i h1[i,i](na_i + nb_i)
+ *   + sum_{i!=j} (K_ij - J_ij)(na_i na_j + nb_i nb_j)
+ *   - sum_ij J_ij (na_i nb_j + nb_i na_j),
+ * J_ij = Re h2[i,j,i,j], K_ij = Re h2[i,j,j,i] (norb x norb, FP64), from
+ * the string tables d_astr / d_bstr (fqehip_string_tables). */
+int fqehip_diagonal(int norb, int64_t lena, int64_t lenb,
+                    const int64_t* d_astr, const int64_t* d_bstr,
+                    const double* d_h1diag, const double* d_jmat,
+                    const double* d_kmat, double* d_diag, void* stream);
+/* d_out[j] = sum_i conj(v_j[i]) x[i], j < k, FP64 with a fixed summation
+ * order; d_workspace holds fqehip_block_dot_workspace_bytes(k) bytes. */
+int64_t fqehip_block_dot_workspace_bytes(int k);
+int fqehip_block_dot(int64_t n, int k, const double* d_v, int64_t ldv,
+                     const double* d_x, double* d_out, double* d_workspace,
+                     void* stream);
+/* x = beta x + sum_j c_j v_j (d_c: k complex128 on the device). */
+int fqehip_block_combine(int64_t n, int k, const double* d_v, int64_t ldv,
+                         const double* d_c, double beta_re, double beta_im,
+                         double* d_x, void* stream);
+/* t[i] = -(1/(diag_i - theta) + corr_i) r[i]; |diag_i - theta| is floored
+ * at eps; d_corr (FP32 [n], the learned correction) may be NULL.  The
+ * reference's update is reciprocal(theta - diag) * residual
+ * (src/fqe/algorithm/davidson.py:282-291). */
+int fqehip_davidson_precondition(int64_t n, const double* d_r,
+                                 const double* d_diag, double theta,
+                                 double eps, const float* d_corr, double* d_t,
+                                 void* stream);
+
+/* ---------------------------------------------------------------------
+ * Learned correction of the Davidson preconditioner (no counterpart in the
+ * reference, whose preconditioner is the plain diagonal,
+ * src/fqe/algorithm/davidson.py:282-291).  A 3-layer MLP
+ * 10 -> 128 -> 128 -> 1 (tanh), FP32, parameters W1 [128][10], b1 [128],
+ * W2 [128][128], b2 [128], w3 [128], b3 [1].  Features of determinant I
+ * at Ritz value theta (d = <I|H|I>, dmin = min d, reference strings ref_a,
+ * ref_b with their highest occupied / lowest empty orbitals):
+ *   d - theta, 1/(|d - theta| + 0.05), d - dmin, theta - dmin,
+ *   popcount(Ia & ~ref_a), popcount(Ib & ~ref_b),
+ *   Ia[homo_a], Ia[lumo_a], Ib[homo_b], Ib[lumo_b].
+ * ------------------------------------------------------------------- */
+#define FQEHIP_MLP_NFEAT 10
+#define FQEHIP_MLP_WIDTH 128
+#define FQEHIP_MLP_NPARAMS 18049
+/* x[s][f] for the determinants d_idx[s] (flat Ia*lenb+Ib), s < m. */
+int fqehip_mlp_features(int norb, int64_t lena, int64_t lenb,
+                        const int64_t* d_astr, const int64_t* d_bstr,
+                        const double* d_diag, double theta, double dmin,
+                        int64_t ref_a, int64_t ref_b, int64_t m,
+                        const int64_t* d_idx, float* d_x, void* stream);
+/* Inference for every determinant: d_corr[I] (FP32); d_xwork holds the
+ * features, lena*lenb*FQEHIP_MLP_NFEAT floats. */
+int fqehip_mlp_correction(const float* d_params, int norb, int64_t lena,
+                          int64_t lenb, const int64_t* d_astr,
+                          const int64_t* d_bstr, const double* d_diag,
+                          double theta, double dmin, int64_t ref_a,
+                          int64_t ref_b, float* d_xwork, float* d_corr,
+                          void* stream);
+/* Network output of m feature rows. */
+int fqehip_mlp_forward(const float* d_params, int64_t m, const float* d_x,
+                       float* d_y, void* stream);
+/* Gradient of L = sum_s w_s (y_s - t_s)^2 over m samples:
+ * d_grad[0..NPARAMS) = dL/dparams, d_grad[NPARAMS] = L (fixed summation
+ * order).  d_workspace: fqehip_mlp_train_workspace_floats(m) floats. */
+int64_t fqehip_mlp_train_workspace_floats(int64_t m);
+int fqehip_mlp_loss_grad(const float* d_params, int64_t m, const float* d_x,
+                         const float* d_target, const float* d_weight,
+                         float* d_grad, float* d_workspace, void* stream);
+/* One Adam step (bias-corrected, step >= 1) on the parameter vector. */
+int fqehip_mlp_adam(float* d_params, const float* d_grad, float* d_m,
+                    float* d_v, int step, float lr, float beta1, float beta2,
+                    float eps, void* stream);
 
 #ifdef __cplusplus
 }

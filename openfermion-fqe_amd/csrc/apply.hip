@@ -387,11 +387,17 @@ int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
 
 // Defined in fused.hip.
 bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb);
-int launch_fused(int precision, int npair, int64_t lena, int64_t lenb,
+bool split_supported(int npair, int64_t lena, int64_t lenb);
+int launch_fused(int precision, int norb, int npair, int64_t lena, int64_t lenb,
                  const double* a, const int32_t* pta, const int32_t* ptb,
                  const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
                  int nslot, const void* coeff, void* slots, void* rowacc,
-                 hipStream_t st, const int32_t* done);
+                 bool split, hipStream_t st, const int32_t* done);
+// Defined in beta_block.hip.
+bool beta_gemm_shape_ok(int64_t lena, int64_t lenb, int64_t ldh);
+int launch_beta_gemm(int64_t lena, int64_t lenb, int64_t ldh, const void* hhi,
+                     const void* hlo, const float* coeff, float* out,
+                     hipStream_t st, const int32_t* done);
 int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
                        const void* slots, const void* rowacc, double ar,
                        double ai, double br, double bi, const void* cin,
@@ -605,9 +611,23 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
   const int64_t n = s->lena * s->lenb;
   char* ws = static_cast<char*>(d_workspace);
   void* rowacc = ws + static_cast<int64_t>(s->nslot) * 2 * n * elem_size(precision);
-  return launch_fused(precision, s->npair, s->lena, s->lenb, op->a, s->ptab_a,
-                      s->ptab_b, s->fwd_a, s->fwd_b, s->slot_a, s->nslot,
-                      d_coeff, ws, rowacc, as_stream(stream), d_done);
+  hipStream_t st = as_stream(stream);
+  const bool split = precision == FQEHIP_C64 && s->hbb_hi != nullptr;
+  if (split) {
+    FQEHIP_CHECK_ARG(s->hbb_lo != nullptr, "hbb_hi set without hbb_lo");
+    FQEHIP_CHECK_ARG(beta_gemm_shape_ok(s->lena, s->lenb, s->ldh) &&
+                         split_supported(s->npair, s->lena, s->lenb),
+                     "the split apply does not support this sector (ldh=%lld)",
+                     static_cast<long long>(s->ldh));
+    // rowacc <- C Hbb^T, then the alpha rows add their diagonal part
+    int rc = launch_beta_gemm(s->lena, s->lenb, s->ldh, s->hbb_hi, s->hbb_lo,
+                              static_cast<const float*>(d_coeff),
+                              static_cast<float*>(rowacc), st, d_done);
+    if (rc) return rc;
+  }
+  return launch_fused(precision, s->norb, s->npair, s->lena, s->lenb, op->a,
+                      s->ptab_a, s->ptab_b, s->fwd_a, s->fwd_b, s->slot_a,
+                      s->nslot, d_coeff, ws, rowacc, split, st, d_done);
 }
 
 extern "C" int fqehip_apply_slot_reduce(int precision, const fqehip_sector* s,

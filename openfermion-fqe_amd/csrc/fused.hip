@@ -145,16 +145,21 @@ struct RowTables {
   int32_t* ssgn;    // [p] sign of the alpha target
 };
 
+// own_q >= 0 (split kernel): gather row own_q is C[ia] itself, sign +1.
 template <typename T>
 __device__ __forceinline__ void load_row_tables(
     int64_t ia, int K, int64_t lena, int64_t lenb, int nslot,
     const int32_t* __restrict__ pta, const int32_t* __restrict__ fwa,
-    const int32_t* __restrict__ sla, RowTables rt, int tid, int nthreads) {
+    const int32_t* __restrict__ sla, RowTables rt, int tid, int nthreads,
+    int own_q = -1) {
   for (int q = tid; q < kRowTabs; q += nthreads) {
     int32_t off = -1, sg = 0;
     int64_t sb = -1;
     int32_t ss = 0;
-    if (q < K) {
+    if (q == own_q) {
+      off = static_cast<int32_t>(ia * lenb * 2);
+      sg = 1;
+    } else if (q < K) {
       const int32_t a = pta[static_cast<int64_t>(q) * lena + ia];
       if (a) {
         off = static_cast<int32_t>((iabs(a) - 1) * lenb * 2);
@@ -204,17 +209,25 @@ __device__ __forceinline__ void scatter_complex(
 }
 
 // ---------------- mixed precision: bf16 hi/lo x3, FP32 accumulate ---------
-// 8 waves, <= 128 VGPRs: two workgroups (two alpha rows) per CU, so one
-// row's gather overlaps the other's MFMAs and scatter.
+// NW waves, two or more workgroups (alpha rows) per CU, so one row's gather
+// overlaps the other's MFMAs and scatter.
+//
+// kSplit (pair-symmetric operator, beta-beta block done by
+// k_beta_gemm_bf16x3 into rowacc beforehand): the gathered operand is
+// D'[q] = D^a[q] + 2 D^b[q] (q < npair) and D'[npair] = C[ia]; A' has the
+// extra column A'[p][npair] = h1'[p], and its row npair is rebuilt per
+// alpha row as the sum of the rows of the occupied diagonal pairs (ii).
+// The scatter then stores off-diagonal alpha excitations into slot rows
+// and adds row npair to rowacc[ia]: no atomics, no LDS accumulator row.
 constexpr int kBfTbC = 32;                 // complex columns per chunk
 constexpr int kBfKPad = 128;
 constexpr int kBfKSteps = kBfKPad / 32;
 constexpr int kBfStride = kBfKPad + 8;     // bf16 per LDS column (padding)
 constexpr int kBfCols = 2 * kBfTbC;        // re block then im block
 
-template <int NW, int WPE>
+template <int NW, int WPE, bool kSplit>
 __global__ void __launch_bounds__(NW * 64, WPE)
-k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
+k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
                const double* __restrict__ amat,
                const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
                const int32_t* __restrict__ fwa, const int32_t* __restrict__ fwb,
@@ -240,6 +253,8 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
   rt.asgn = rt.aoff + kRowTabs;
   rt.ssgn = rt.asgn + kRowTabs;
   float* accrow = reinterpret_cast<float*>(rt.ssgn + kRowTabs);
+  double* drow = reinterpret_cast<double*>(rt.ssgn + kRowTabs);  // kSplit
+  const int Kg = kSplit ? K + 1 : K;       // gathered rows of the D tile
 
   bf16x8 ahi[kRT][kBfKSteps], alo[kRT][kBfKSteps];
 #pragma unroll
@@ -250,7 +265,13 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int kk = kt * 32 + 8 * (lane >> 4) + j;
-        const double v = (row < M && kk < K) ? amat[static_cast<int64_t>(row) * K + kk] : 0.0;
+        double v = 0.0;
+        if (kSplit) {
+          if (row < npair && kk < K) v = amat[static_cast<int64_t>(row) * K + kk];
+          else if (row < npair && kk == K) v = amat[static_cast<int64_t>(npair) * K + row];
+        } else if (row < M && kk < K) {
+          v = amat[static_cast<int64_t>(row) * K + kk];
+        }
         const __bf16 h = to_bf16(static_cast<float>(v));
         ahi[r][kt][j] = h;
         alo[r][kt][j] = to_bf16(static_cast<float>(v - static_cast<double>(from_bf16(h))));
@@ -267,8 +288,40 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
 
   for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
     load_row_tables<float>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
-                           kT);
-    for (int64_t x = tid; x < 2 * lenb; x += kT) accrow[x] = 0.0f;
+                           kT, kSplit ? K : -1);
+    if (kSplit) {
+      __syncthreads();
+      // row npair of A' for this alpha string: sum over occupied orbitals i
+      // (E^a_ii ia = ia, marked -2 in the row tables) of A'[ii][:]
+      for (int k = tid; k < Kg; k += kT) {
+        double v = 0.0;
+        for (int i = 0; i < norb; ++i) {
+          const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
+          if (rt.sbase[dp] == -2)
+            v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
+                       : amat[static_cast<int64_t>(npair) * K + dp];
+        }
+        drow[k] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int r = 0; r < kRT; ++r) {
+        if ((wave * kRT + r) * 16 + (lane & 15) == npair) {
+#pragma unroll
+          for (int kt = 0; kt < kBfKSteps; ++kt)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const int kk = kt * 32 + 8 * (lane >> 4) + j;
+              const double v = kk < Kg ? drow[kk] : 0.0;
+              const __bf16 h = to_bf16(static_cast<float>(v));
+              ahi[r][kt][j] = h;
+              alo[r][kt][j] = to_bf16(static_cast<float>(v - static_cast<double>(from_bf16(h))));
+            }
+        }
+      }
+    } else {
+      for (int64_t x = tid; x < 2 * lenb; x += kT) accrow[x] = 0.0f;
+    }
     const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(float));
     int32_t bt[kGIter][8];                 // gather-table entries, this chunk
     auto load_bt = [&](int64_t ib0) {
@@ -298,7 +351,7 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
             const int q = q0 + j;
-            const int32_t off = (colok && q < K) ? rt.aoff[q] : -1;
+            const int32_t off = (colok && q < Kg) ? rt.aoff[q] : -1;
             sa[j] = off >= 0 ? static_cast<float>(rt.asgn[q]) : 0.0f;
             ca[j] = bload_f2(rc, off >= 0 ? (off + 2 * ib) * 4 : -1);
             const int32_t b = bt[it][j];
@@ -306,7 +359,7 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
           }
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const float sb = sgnf(bt[it][j]);
+            const float sb = (kSplit ? 2.0f : 1.0f) * sgnf(bt[it][j]);
             const float re = sa[j] * ca[j].x + sb * cb[j].x;
             const float im = sa[j] * ca[j].y + sb * cb[j].y;
             const __bf16 h0 = to_bf16(re), h1 = to_bf16(im);
@@ -335,7 +388,7 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
             for (int i = 0; i < 4; ++i) {
               const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
               const int jb = static_cast<int>(ib0) + h * 16 + (lane & 15);
-              fbv[r][i] = bload_i32(rfb, (p < npair && jb < lenb)
+              fbv[r][i] = kSplit ? 0 : bload_i32(rfb, (p < npair && jb < lenb)
                                              ? (p * static_cast<int>(lenb) + jb) * 4 : -1);
             }
           floatx4 are[kRT], aim[kRT];
@@ -370,16 +423,36 @@ k_fused_bf16x3(int npair, int64_t lena, int64_t lenb,
             for (int i = 0; i < 4; ++i) {
               const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
               const int64_t jb = ib0 + h * 16 + (lane & 15);
-              if (p <= npair && jb < lenb)
+              if (kSplit) {
+                if (jb >= lenb || p > npair) continue;
+                if (p == npair) {
+                  float2* d = reinterpret_cast<float2*>(rowacc + (ia * lenb + jb) * 2);
+                  float2 o = *d;
+                  o.x += are[r][i];
+                  o.y += aim[r][i];
+                  *d = o;
+                } else {
+                  const int64_t sb = rt.sbase[p];
+                  if (sb >= 0) {
+                    const float sg = static_cast<float>(rt.ssgn[p]);
+                    float* dst = slots + sb + 2 * jb;
+                    dst[0] = sg * are[r][i];
+                    dst[1] = sg * aim[r][i];
+                  }
+                }
+              } else if (p <= npair && jb < lenb) {
                 scatter_complex<float, float>(are[r][i], aim[r][i], p, npair,
                                               jb, fbv[r][i], rt, accrow, slots);
+              }
             }
         }
       }
       __syncthreads();
     }
-    float* dst = rowacc + ia * 2 * lenb;
-    for (int64_t x = tid; x < 2 * lenb; x += kT) dst[x] = accrow[x];
+    if (!kSplit) {
+      float* dst = rowacc + ia * 2 * lenb;
+      for (int64_t x = tid; x < 2 * lenb; x += kT) dst[x] = accrow[x];
+    }
     __syncthreads();
   }
 }
@@ -596,9 +669,9 @@ size_t row_tables_bytes() {
   return kRowTabs * (sizeof(int64_t) + 3 * sizeof(int32_t));
 }
 
-size_t bf16_lds_bytes(int64_t lenb) {
+size_t bf16_lds_bytes(int64_t lenb, bool split = false) {
   return 2 * kBfCols * kBfStride * sizeof(__bf16) + row_tables_bytes() +
-         2 * lenb * sizeof(float);
+         (split ? kBfKPad * sizeof(double) : 2 * lenb * sizeof(float));
 }
 
 size_t f64_lds_bytes(int64_t lenb) {
@@ -637,12 +710,30 @@ bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb) {
          lena * lenb * 2 * 4 < lim;
 }
 
-int launch_fused(int precision, int npair, int64_t lena, int64_t lenb,
-                 const double* a, const int32_t* pta, const int32_t* ptb,
-                 const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
-                 int nslot, const void* coeff, void* slots, void* rowacc,
-                 hipStream_t st, const int32_t* done) {
+bool split_supported(int npair, int64_t lena, int64_t lenb) {
+  return npair + 1 <= kBfKPad && fused_supported(FQEHIP_C64, npair, lena, lenb) &&
+         bf16_lds_bytes(lenb, true) <= kLdsLimit;
+}
+
+int launch_fused(int precision, int norb, int npair, int64_t lena,
+                 int64_t lenb, const double* a, const int32_t* pta,
+                 const int32_t* ptb, const int32_t* fwa, const int32_t* fwb,
+                 const int32_t* sla, int nslot, const void* coeff, void* slots,
+                 void* rowacc, bool split, hipStream_t st, const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
+  if (split) {
+    if (precision != FQEHIP_C64 || !split_supported(npair, lena, lenb)) {
+      set_error("the split apply needs complex64 and npair + 1 <= %d", kBfKPad);
+      return FQEHIP_ERR_INVALID;
+    }
+    k_fused_bf16x3<4, 2, true><<<fused_grid(lena), 256, bf16_lds_bytes(lenb, true), st>>>(
+        norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
+        static_cast<const float*>(coeff), static_cast<float*>(slots),
+        static_cast<float*>(rowacc), done);
+    FQEHIP_CHECK_LAUNCH("k_fused_bf16x3_split");
+    return FQEHIP_OK;
+  }
   if (precision == FQEHIP_C128) {
     const size_t lds = f64_lds_bytes(lenb);
     k_fused_f64<<<fused_grid(lena), kFThreads, lds, st>>>(
@@ -655,13 +746,13 @@ int launch_fused(int precision, int npair, int64_t lena, int64_t lenb,
     // variant 4: 4 waves x 2 row tiles, two workgroups per CU;
     // variant 8: 8 waves x 1 row tile, one workgroup per CU
     if (bf16_variant() == 8)
-      k_fused_bf16x3<8, 2><<<fused_grid(lena), 512, lds, st>>>(
-          npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
+      k_fused_bf16x3<8, 2, false><<<fused_grid(lena), 512, lds, st>>>(
+          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
           static_cast<const float*>(coeff), static_cast<float*>(slots),
           static_cast<float*>(rowacc), done);
     else
-      k_fused_bf16x3<4, 2><<<fused_grid(lena), 256, lds, st>>>(
-          npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
+      k_fused_bf16x3<4, 2, false><<<fused_grid(lena), 256, lds, st>>>(
+          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
           static_cast<const float*>(coeff), static_cast<float*>(slots),
           static_cast<float*>(rowacc), done);
     FQEHIP_CHECK_LAUNCH("k_fused_bf16x3");

@@ -37,7 +37,8 @@ class Sector(ctypes.Structure):
                 ("npair", c_int), ("lena", c_int64), ("lenb", c_int64),
                 ("ptab_a", c_void_p), ("ptab_b", c_void_p),
                 ("fwd_a", c_void_p), ("fwd_b", c_void_p),
-                ("slot_a", c_void_p), ("nslot", c_int)]
+                ("slot_a", c_void_p), ("nslot", c_int),
+                ("hbb_hi", c_void_p), ("hbb_lo", c_void_p), ("ldh", c_int64)]
 
 
 class Operator(ctypes.Structure):
@@ -86,6 +87,41 @@ _SIGNATURES = {
                                          c_void_p, c_void_p, c_void_p,
                                          c_void_p]),
     "fqehip_apply_workspace_bytes": (c_int64, [c_int, POINTER(Sector)]),
+    "fqehip_beta_block_ld": (c_int64, [c_int64]),
+    "fqehip_diagonal": (c_int, [c_int, c_int64, c_int64, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p]),
+    "fqehip_block_dot_workspace_bytes": (c_int64, [c_int]),
+    "fqehip_block_dot": (c_int, [c_int64, c_int, c_void_p, c_int64, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
+    "fqehip_block_combine": (c_int, [c_int64, c_int, c_void_p, c_int64,
+                                     c_void_p, c_double, c_double, c_void_p,
+                                     c_void_p]),
+    "fqehip_mlp_features": (c_int, [c_int, c_int64, c_int64, c_void_p,
+                                    c_void_p, c_void_p, c_double, c_double,
+                                    c_int64, c_int64, c_int64, c_void_p,
+                                    c_void_p, c_void_p]),
+    "fqehip_mlp_correction": (c_int, [c_void_p, c_int, c_int64, c_int64,
+                                      c_void_p, c_void_p, c_void_p, c_double,
+                                      c_double, c_int64, c_int64, c_void_p,
+                                      c_void_p, c_void_p]),
+    "fqehip_mlp_forward": (c_int, [c_void_p, c_int64, c_void_p, c_void_p,
+                                   c_void_p]),
+    "fqehip_mlp_train_workspace_floats": (c_int64, [c_int64]),
+    "fqehip_mlp_loss_grad": (c_int, [c_void_p, c_int64, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fqehip_mlp_adam": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, c_void_p]),
+    "fqehip_davidson_precondition": (c_int, [c_int64, c_void_p, c_void_p,
+                                             c_double, c_double, c_void_p,
+                                             c_void_p, c_void_p]),
+    "fqehip_beta_block": (c_int, [c_int, c_int64, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_int64,
+                                  c_void_p]),
+    "fqehip_beta_gemm": (c_int, [c_int64, c_int64, c_void_p, c_void_p,
+                                 c_int64, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
     "fqehip_apply": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
                              c_void_p, c_double, c_double, c_double, c_double,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
@@ -133,7 +169,7 @@ def ptr(t) -> int:
 
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
-    if name.endswith(("_count", "_bytes", "_supported")):
+    if name.endswith(("_count", "_bytes", "_supported", "_ld", "_floats")):
         return rc
     if rc != OK:
         msg = lib().fqehip_last_error().decode(errors="replace")

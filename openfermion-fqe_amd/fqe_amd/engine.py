@@ -37,6 +37,34 @@ class PreparedOperator:
         self.a = a
         self.h1 = h1
         self.npair = norb * (norb + 1) // 2 if compressed else norb * norb
+        # A[p][q] == A[q][p] exactly (p, q < npair): the split apply applies
+        self.pair_symmetric = False
+        self._hbb = {}
+
+    def beta_block(self, graph):
+        """bfloat16 hi/lo split of the beta-beta block of this operator on
+        the beta strings of ``graph`` (fqehip_beta_block), cached per
+        (nbeta, device); None when the split apply does not apply."""
+        if not self.pair_symmetric or self.a is None or self.complex_op:
+            return None
+        lenb = graph.lenb()
+        if lenb * 8 > 160 * 1024:
+            return None
+        key = (graph.nbeta(), self.a.device.index)
+        if key not in self._hbb:
+            fwb, _ = graph.tables(False).fused_tables(self.compressed)
+            ldh = int(_lib.call("fqehip_beta_block_ld", lenb))
+            hcol = torch.empty((lenb, lenb), dtype=torch.float64,
+                               device=self.a.device)
+            hhi = torch.empty((ldh, ldh), dtype=torch.int16,
+                              device=self.a.device)
+            hlo = torch.empty_like(hhi)
+            _lib.call("fqehip_beta_block", self.npair, lenb, _lib.ptr(fwb),
+                      _lib.ptr(self.a), _lib.ptr(hcol), _lib.ptr(hhi),
+                      _lib.ptr(hlo), ldh, _lib.stream_ptr())
+            self._hbb[key] = (hhi, hlo, ldh, fwb)
+            del hcol
+        return self._hbb[key]
 
     def c_struct(self) -> _lib.Operator:
         return _lib.Operator(int(self.complex_op), _lib.ptr(self.a),
@@ -101,7 +129,11 @@ def prepare_operator(tensors: Sequence[numpy.ndarray], norb: int,
         a = numpy.ascontiguousarray(full.real)
     dev_a = torch.from_numpy(numpy.ascontiguousarray(a, dtype=numpy.float64)
                              ).to(device)
-    return PreparedOperator(norb, compressed, complex_op, dev_a, None)
+    prepared = PreparedOperator(norb, compressed, complex_op, dev_a, None)
+    if not complex_op:
+        sq = a[:-1, :]
+        prepared.pair_symmetric = bool(numpy.array_equal(sq, sq.T))
+    return prepared
 
 
 def precision_of(dtype: torch.dtype) -> int:
@@ -112,9 +144,13 @@ def precision_of(dtype: torch.dtype) -> int:
     raise TypeError(f"unsupported coefficient dtype {dtype}")
 
 
-def sector_struct(graph, op: PreparedOperator) -> Tuple[_lib.Sector, tuple]:
+def sector_struct(graph, op: PreparedOperator,
+                  precision: Optional[int] = None
+                  ) -> Tuple[_lib.Sector, tuple]:
     """C descriptor of a sector for the pair basis of ``op``; the second
-    element keeps the tables referenced."""
+    element keeps the tables referenced.  For complex64 and a
+    pair-symmetric operator the beta-beta block is attached (split apply,
+    ``settings.split``)."""
     ta = graph.tables(True).pair_table(op.compressed)
     tb = graph.tables(False).pair_table(op.compressed)
     keep = [ta, tb]
@@ -124,9 +160,17 @@ def sector_struct(graph, op: PreparedOperator) -> Tuple[_lib.Sector, tuple]:
         fwb, _ = graph.tables(False).fused_tables(op.compressed)
         keep += [fwa, fwb, sla]
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
+    hbb = None
+    if (fwa is not None and settings.split and precision == _lib.C64):
+        hbb = op.beta_block(graph)
     s = _lib.Sector(graph.norb(), graph.nalpha(), graph.nbeta(), op.npair,
                     graph.lena(), graph.lenb(), _lib.ptr(ta), _lib.ptr(tb),
-                    _lib.ptr(fwa), _lib.ptr(fwb), _lib.ptr(sla), nslot)
+                    _lib.ptr(fwa), _lib.ptr(fwb), _lib.ptr(sla), nslot,
+                    _lib.ptr(hbb[0]) if hbb else None,
+                    _lib.ptr(hbb[1]) if hbb else None,
+                    hbb[2] if hbb else 0)
+    if hbb:
+        keep += [hbb[0], hbb[1]]
     return s, tuple(keep)
 
 
@@ -159,7 +203,7 @@ def apply_sector(graph, coeff: torch.Tensor, op: PreparedOperator,
                  ws: Optional[torch.Tensor] = None) -> None:
     """out = alpha * H coeff + beta * cin on one sector (device tensors)."""
     prec = precision_of(coeff.dtype)
-    s, keep = sector_struct(graph, op)
+    s, keep = sector_struct(graph, op, prec)
     if ws is None and op.h1 is None:
         ws = workspace(s, prec, coeff.device)
     opc = op.c_struct()

@@ -17,3 +17,8 @@ max_norb = 32
 # Use the fused gather + GEMM + scatter kernels for real operators when the
 # sector fits them (else, and for complex operators, the unfused kernels).
 fused = True
+
+# complex64 applies of a pair-symmetric real operator: beta-beta block as
+# one BF16x3 GEMM and an alpha-only row kernel (no beta scatter).  Off, or
+# for other operators, the row kernel also scatters beta excitations.
+split = True

@@ -285,7 +285,8 @@ def _taylor(wfn: Wavefunction, time: float, hamil: RestrictedHamiltonian,
         c = sec.device_coeff()
         terms.append(c.clone())
         results.append(torch.view_as_real(c.to(torch.complex128).clone()))
-        s, keep = engine.sector_struct(sec.get_fcigraph(), op)
+        s, keep = engine.sector_struct(sec.get_fcigraph(), op,
+                                       engine.precision_of(c.dtype))
         structs.append((s, keep))
         wss.append(engine.workspace(s, engine.precision_of(c.dtype), dev))
         offs.append(offs[-1] + engine.norm_partials_count(c.numel()))
@@ -334,7 +335,7 @@ def _taylor_single(out: Wavefunction, sec: FqeData,
     prec = engine.precision_of(c.dtype)
     term = c.clone()
     result = c.to(torch.complex128).clone()
-    s, keep = engine.sector_struct(sec.get_fcigraph(), op)
+    s, keep = engine.sector_struct(sec.get_fcigraph(), op, prec)
     ws = engine.workspace(s, prec, dev)
     partials = torch.zeros(max(engine.norm_partials_count(c.numel()), 1),
                            dtype=torch.float64, device=dev)

@@ -12,6 +12,11 @@ test data.  Nothing of the reference is imported or executed:
   ``ast.literal_eval`` on their literal nodes: strings and addresses of
   ``tests/fci_graph_test.py:33-112``, the n=4 maps of ``:151-190``, and the
   eigenvector / eigenvalue of ``tests/fqe_data_test.py:761-826``.
+* the LiH (norb=6, 2a2b) h1e, h2e and ground-state wavefunction literals of
+  ``tests/unittest_data/build_lih_data.py:30-1270`` (energy branch), whose
+  FCI energy -8.877719570384043 (``tests/davidson_test.py:27``) and HF
+  energy -8.857341498221992 (``tests/wavefunction_test.py:537-548``) the
+  Davidson tests use.
 
 Run from the repository root:  python tests/golden/make_fixtures.py
 """
@@ -88,8 +93,19 @@ def literal_tables():
     }
 
 
+def lih_arrays():
+    with open(os.path.join(REF, "unittest_data", "build_lih_data.py")) as f:
+        tree = ast.parse(f.read())
+    out = {}
+    for var in ("wfn", "h1e", "h2e"):
+        out[var] = numpy.asarray(_literal(_find_assign(tree, "build_lih_data",
+                                                       var)))
+    return out
+
+
 def main():
     numpy.savez(os.path.join(OUT, "fqe_data_golden.npz"), **fqe_data_arrays())
+    numpy.savez(os.path.join(OUT, "lih_energy.npz"), **lih_arrays())
     with open(os.path.join(OUT, "literal_tables.json"), "w") as f:
         json.dump(literal_tables(), f, indent=1)
 

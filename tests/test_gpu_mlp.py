@@ -1,0 +1,140 @@
+"""GPU tests of the learned Davidson preconditioner: features, forward
+pass and gradients of the hand-written kernels against the CPU oracle
+(oracle/mlp_oracle.py, itself pinned by finite differences), training on
+the GPU, and what the trained network gains over the plain diagonal
+preconditioner."""
+
+import numpy
+import pytest
+
+from oracle import fqe_oracle as fo
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL = 2e-4     # FP32 kernels vs float64 oracle, relative to max |g|
+
+
+@pytest.fixture(scope="module")
+def fqe():
+    import torch
+    assert torch.cuda.is_available()
+    import fqe_amd
+    return fqe_amd
+
+
+def test_parameter_layout_matches_oracle(fqe):
+    from fqe_amd.algorithm import mlp_precond as mp
+    assert mp.NPARAMS == mo.nparams() == 18049
+    assert mp.NFEAT == mo.NFEAT and mp.WIDTH == mo.WIDTH
+
+
+@pytest.mark.parametrize("na,nb,norb", [(2, 2, 6), (3, 1, 5), (0, 2, 4)])
+def test_features_match_oracle(fqe, na, nb, norb):
+    import torch
+    from fqe_amd.algorithm import mlp_precond as mp
+    rng = numpy.random.default_rng(norb + 10 * na)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    graph = fqe.FciGraph(na, nb, norb)
+    feats = mp.SectorFeatures(ham, graph)
+    theta = feats.dmin - 0.3
+    idx = torch.arange(feats.n, device=feats.diag.device)
+    x = feats.features(theta, idx).cpu().numpy()
+    g = fo.OracleGraph(na, nb, norb)
+    ref = mo.features(g.astr, g.bstr, feats.diag.cpu().numpy(), theta,
+                      feats.dmin, feats.ref_a, feats.ref_b, norb)
+    assert numpy.allclose(x, ref, rtol=1e-6, atol=1e-6)
+
+
+def test_forward_and_gradient_match_oracle(fqe):
+    import torch
+    from fqe_amd import _lib
+    from fqe_amd.algorithm import mlp_precond as mp
+    rng = numpy.random.default_rng(7)
+    params = mo.init_params(3)
+    params = (params + 0.05 * rng.normal(size=params.size)).astype(numpy.float32)
+    m = 777
+    x = rng.normal(size=(m, mo.NFEAT)).astype(numpy.float32)
+    t = rng.normal(size=m).astype(numpy.float32)
+    w = rng.uniform(0.0, 1.0, size=m).astype(numpy.float32)
+    w /= w.sum()
+    model = mp.MLPPreconditioner(params)
+    dev = _lib.device()
+    xd = torch.from_numpy(x).to(dev)
+    y = model.forward(xd).cpu().numpy()
+    yref = mo.forward(params.astype(numpy.float64), x.astype(numpy.float64))
+    assert numpy.max(numpy.abs(y - yref)) < 1e-5 * max(1.0, numpy.abs(yref).max())
+    g = model.loss_grad(xd, torch.from_numpy(t).to(dev),
+                        torch.from_numpy(w).to(dev)).cpu().numpy()
+    loss, gref = mo.loss_and_grad(params.astype(numpy.float64),
+                                  x.astype(numpy.float64),
+                                  t.astype(numpy.float64),
+                                  w.astype(numpy.float64))
+    scale = numpy.abs(gref).max()
+    assert numpy.max(numpy.abs(g[:-1] - gref)) < GRAD_TOL * scale
+    assert abs(g[-1] - loss) < 1e-5 * max(1.0, loss)
+
+
+def test_adam_matches_oracle(fqe):
+    import torch
+    from fqe_amd import _lib
+    from fqe_amd.algorithm import mlp_precond as mp
+    rng = numpy.random.default_rng(9)
+    p = rng.normal(size=mp.NPARAMS).astype(numpy.float32)
+    m0 = numpy.zeros(mp.NPARAMS)
+    v0 = numpy.zeros(mp.NPARAMS)
+    dev = _lib.device()
+    pd = torch.from_numpy(p.copy()).to(dev)
+    md = torch.zeros(mp.NPARAMS, dtype=torch.float32, device=dev)
+    vd = torch.zeros_like(md)
+    pr = p.astype(numpy.float64)
+    for step in range(1, 4):
+        g = rng.normal(size=mp.NPARAMS).astype(numpy.float32)
+        gd = torch.from_numpy(numpy.append(g, 0.0).astype(numpy.float32)).to(dev)
+        _lib.call("fqehip_mlp_adam", _lib.ptr(pd), _lib.ptr(gd), _lib.ptr(md),
+                  _lib.ptr(vd), step, 1e-3, 0.9, 0.999, 1e-8, _lib.stream_ptr())
+        pr, m0, v0 = mo.adam_step(pr, g.astype(numpy.float64), m0, v0, step,
+                                  1e-3)
+    assert numpy.max(numpy.abs(pd.cpu().numpy() - pr)) < 1e-6
+
+
+def test_training_and_gain(fqe):
+    """Train on the method's own correction vectors of small synthetic
+    Hamiltonians (norb=6, 3a3b) and compare Davidson iterations to a 1e-8
+    residual on held-out ones; the learned run must converge, and its
+    iteration count is reported beside the plain one."""
+    import torch
+    from fqe_amd.algorithm import mlp_precond as mp
+    xs, ts, ws = [], [], []
+    for seed in range(24):
+        rng = numpy.random.default_rng(1000 + seed)
+        h1, h2 = fo.synthetic_hamiltonian(6, 6, rng)
+        ham = fqe.get_restricted_hamiltonian((h1, h2))
+        out = mp.correction_samples(ham, 3, 3, per_snapshot=128, seed=seed)
+        if out is None:
+            continue
+        xs.append(out[0])
+        ts.append(out[1])
+        ws.append(out[2])
+    x = torch.cat(xs)
+    t = torch.cat(ts)
+    w = torch.cat(ws)
+    model = mp.MLPPreconditioner(seed=0)
+    losses = model.train(x, t, w, epochs=40, batch=1024, lr=2e-3)
+    print(f"training: {x.shape[0]} samples, loss {losses[0]:.4e} -> "
+          f"{losses[-1]:.4e}", flush=True)
+    assert losses[-1] < losses[0]
+    plain_it, learned_it = [], []
+    for seed in range(6):
+        rng = numpy.random.default_rng(5000 + seed)
+        h1, h2 = fo.synthetic_hamiltonian(6, 6, rng)
+        ham = fqe.get_restricted_hamiltonian((h1, h2))
+        r = mp.solve_both(ham, 3, 3, model, tol=1e-8)
+        assert r["plain"].converged and r["learned"].converged
+        assert abs(r["plain"].eigenvalues[0] - r["learned"].eigenvalues[0]) < 1e-9
+        plain_it.append(r["plain"].iterations)
+        learned_it.append(r["learned"].iterations)
+    print(f"held-out iterations plain {plain_it} learned {learned_it} "
+          f"(mean {numpy.mean(plain_it):.1f} -> {numpy.mean(learned_it):.1f})",
+          flush=True)

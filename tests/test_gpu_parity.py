@@ -367,8 +367,9 @@ def test_fused_and_unfused_paths(fqe, fused, dtype, na, nb, norb, kind):
     c = fo.synthetic_coeff(g.lena, g.lenb, rng)
     ref = fo.apply_12(c, h1, h2, g)
     npdt = numpy.complex128 if dtype == "c128" else numpy.complex64
-    saved = settings.fused
+    saved = settings.fused, settings.split
     settings.fused = fused
+    settings.split = False   # the split apply has its own tests below
     try:
         d = fqe.FqeData(na, nb, norb, dtype=npdt)
         d.coeff = c
@@ -380,7 +381,7 @@ def test_fused_and_unfused_paths(fqe, fused, dtype, na, nb, norb, kind):
         assert bool(took) == fused
         out = d.apply((h1, h2)).coeff
     finally:
-        settings.fused = saved
+        settings.fused, settings.split = saved
     tol = FP64_TOL if dtype == "c128" else MIXED_TOL
     assert rel(out, ref) < tol
 
@@ -398,3 +399,103 @@ def test_fused_taylor_with_e0(fqe):
                                                                e_0=0.3 + 0.1j))
     assert out._taylor_terms == nterms
     assert rel(out.get_coeff(key), ref) < FP64_TOL
+
+
+def test_fused_taylor_mixed_split(fqe):
+    """Mixed-precision time evolution through the split apply."""
+    norb, na, nb = 9, 4, 3
+    c, h1, h2 = fo.synthetic_workload(norb, na, nb, 8, norb)
+    g = fo.OracleGraph(na, nb, norb)
+    ref, nterms = fo.taylor_evolve(c, (h1, h2), 0.3 + 0.1j, 0.05, g)
+    key = (na + nb, na - nb)
+    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]], dtype=numpy.complex64)
+    wfn.set_wfn(strategy="from_data", raw_data={key: c})
+    out = wfn.time_evolve(0.05, fqe.get_restricted_hamiltonian((h1, h2),
+                                                               e_0=0.3 + 0.1j))
+    err = rel(out.get_coeff(key), ref)
+    print(f"split mixed evolve rel err {err:.3e}", flush=True)
+    assert err < MIXED_TOL
+
+
+# --------------------------------------------------------------------------
+# split apply (complex64, pair-symmetric operator): beta-beta block as a
+# dense GEMM, alpha-only row kernel
+# --------------------------------------------------------------------------
+
+def _bf16_to_f64(t):
+    """int16 tensor of bfloat16 bit patterns -> float64 numpy."""
+    u = t.cpu().numpy().view(numpy.uint16).astype(numpy.uint32) << 16
+    return u.view(numpy.float32).astype(numpy.float64)
+
+
+@pytest.mark.parametrize("nb,norb,kind", [(3, 7, "symmetric"),
+                                          (1, 3, "symmetric"),
+                                          (4, 9, "symmetric"),
+                                          (2, 5, "real"),
+                                          (0, 4, "symmetric"),
+                                          (5, 5, "symmetric")])
+def test_beta_block_matches_dense_sector(fqe, nb, norb, kind):
+    """Hbb equals the dense CI matrix of the (0, nb) sector, in which no
+    alpha string exists to excite (lena = 1); its bf16 hi + lo split
+    represents it to 2^-16 relative and is zero-padded."""
+    import torch
+    from fqe_amd import _lib, engine
+    print(f"beta block nb={nb} norb={norb} {kind}", flush=True)
+    rng = numpy.random.default_rng(100 + 7 * nb + norb)
+    h1, h2 = _h(norb, rng, kind)
+    dense = fo.dense_matrix(h1, h2, fo.OracleGraph(0, nb, norb)).real
+    graph = fqe.FciGraph(0, nb, norb)
+    op = engine.prepare_operator((h1, h2), norb, _lib.device())
+    assert op.pair_symmetric == (kind == "symmetric")
+    lenb = graph.lenb()
+    fwb, _ = graph.tables(False).fused_tables(op.compressed)
+    ldh = int(_lib.call("fqehip_beta_block_ld", lenb))
+    assert ldh % 128 == 0 and ldh >= lenb
+    dev = _lib.device()
+    hcol = torch.full((lenb, lenb), 7.0, dtype=torch.float64, device=dev)
+    hhi = torch.full((ldh, ldh), 77, dtype=torch.int16, device=dev)
+    hlo = torch.full((ldh, ldh), 77, dtype=torch.int16, device=dev)
+    _lib.call("fqehip_beta_block", op.npair, lenb, _lib.ptr(fwb),
+              _lib.ptr(op.a), _lib.ptr(hcol), _lib.ptr(hhi), _lib.ptr(hlo),
+              ldh, _lib.stream_ptr())
+    hbb = hcol.cpu().numpy().T
+    assert rel(hbb, dense) < FP64_TOL
+    split = _bf16_to_f64(hhi) + _bf16_to_f64(hlo)
+    assert numpy.all(split[lenb:, :] == 0) and numpy.all(split[:, lenb:] == 0)
+    err = numpy.abs(split[:lenb, :lenb] - dense)
+    assert numpy.all(err <= 2.0 ** -16 * numpy.abs(dense) + 1e-30)
+
+
+@pytest.mark.parametrize("na,nb,norb", [(3, 0, 7), (0, 3, 7), (5, 5, 10),
+                                        (7, 3, 10), (1, 1, 3), (4, 4, 9),
+                                        (2, 6, 8), (6, 5, 12)])
+def test_split_apply(fqe, na, nb, norb):
+    """The split apply is taken for complex64 and a pair-symmetric operator
+    and agrees with the oracle and with the row kernel's own result."""
+    import ctypes
+    from fqe_amd import _lib, engine, settings
+    print(f"split apply {na} {nb} {norb}", flush=True)
+    rng = numpy.random.default_rng(norb * 13 + na * 5 + nb)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    ref = fo.apply_12(c, h1, h2, g)
+    d = fqe.FqeData(na, nb, norb, dtype=numpy.complex64)
+    d.coeff = c
+    op = engine.prepare_operator((h1, h2), norb, _lib.device())
+    assert op.pair_symmetric
+    s, keep = engine.sector_struct(d.get_fcigraph(), op, _lib.C64)
+    assert s.hbb_hi and s.hbb_lo and s.ldh >= g.lenb
+    assert _lib.lib().fqehip_fused_supported(_lib.C64, ctypes.byref(s),
+                                             ctypes.byref(op.c_struct()))
+    split = d.apply((h1, h2)).coeff
+    saved = settings.split
+    settings.split = False
+    try:
+        rows = d.apply((h1, h2)).coeff
+    finally:
+        settings.split = saved
+    err = rel(split, ref)
+    print(f"split rel err {err:.3e} vs rows {rel(rows, ref):.3e}", flush=True)
+    assert err < MIXED_TOL
+    assert rel(split, rows) < MIXED_TOL

@@ -243,3 +243,76 @@ def test_slater_condon_matches_dense_diagonal(na, nb, norb):
     diag = fo.slater_condon_diagonal(h1, h2, g)
     numpy.testing.assert_allclose(diag.reshape(-1), dense.diagonal().real,
                                   atol=1e-11)
+
+
+# --------------------------------------------------------------------------
+# learned-preconditioner oracle (oracle/mlp_oracle.py)
+# --------------------------------------------------------------------------
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """Back-propagation of the oracle against central differences in
+    float64 over every parameter block (a transposed weight or a dropped
+    tanh' factor fails it)."""
+    from oracle import mlp_oracle as mo
+    rng = numpy.random.default_rng(0)
+    theta = mo.init_params(1).astype(numpy.float64)
+    theta += 0.05 * rng.normal(size=theta.size)
+    x = rng.normal(size=(7, mo.NFEAT))
+    t = rng.normal(size=7)
+    w = rng.uniform(0.1, 1.0, size=7)
+    loss, grad = mo.loss_and_grad(theta, x, t, w)
+    shapes = mo.param_shapes()
+    off = 0
+    for name, shape in shapes:
+        size = int(numpy.prod(shape))
+        picks = rng.choice(size, size=min(size, 12), replace=False)
+        for q in picks:
+            e = numpy.zeros_like(theta)
+            e[off + q] = 1e-6
+            num = (mo.loss_and_grad(theta + e, x, t, w)[0] -
+                   mo.loss_and_grad(theta - e, x, t, w)[0]) / 2e-6
+            assert abs(num - grad[off + q]) < 1e-6 * max(1.0, abs(num)), (name, q)
+        off += size
+    assert abs(loss - numpy.sum(w * (mo.forward(theta, x) - t) ** 2)) < 1e-12
+
+
+def test_mlp_oracle_features_by_hand():
+    """Features of two determinants of the norb=4, 2a1b sector, written out
+    by hand."""
+    from oracle import mlp_oracle as mo
+    g = fo.OracleGraph(2, 1, 4)
+    diag = numpy.arange(g.lena * g.lenb, dtype=float).reshape(g.lena, g.lenb)
+    x = mo.features(g.astr, g.bstr, diag, theta=-0.5, dmin=0.0, ref_a=0b0011,
+                    ref_b=0b0001, norb=4).reshape(g.lena, g.lenb, -1)
+    # alpha strings: 0b0011, 0b0101, ...; beta strings 0b0001, 0b0010, ...
+    ia = list(g.astr).index(0b0101)
+    ib = list(g.bstr).index(0b0010)
+    d = diag[ia, ib]
+    expect = [d + 0.5, 1.0 / (d + 0.5 + 0.05), d, -0.5,
+              1.0, 1.0,          # excitation levels
+              0.0, 1.0,          # alpha HOMO (1) empty, LUMO (2) occupied
+              0.0, 1.0]          # beta HOMO (0) empty, LUMO (1) occupied
+    assert numpy.allclose(x[ia, ib], expect)
+    assert numpy.allclose(x[0, 0, 4:], [0, 0, 1, 0, 1, 0])
+
+
+def test_mlp_oracle_targets_and_adam():
+    from oracle import mlp_oracle as mo
+    rng = numpy.random.default_rng(3)
+    psi = rng.normal(size=6) + 1j * rng.normal(size=6)
+    u = psi + 0.1 * (rng.normal(size=6) + 1j * rng.normal(size=6))
+    r = rng.normal(size=6) + 1j * rng.normal(size=6)
+    inv = rng.uniform(0.2, 1.0, size=6)
+    tgt, w = mo.targets(u, r, psi, inv, clip=1e9)
+    tstar = psi / numpy.vdot(u, psi) - u
+    # the target multiplier minimises |m r_I + t*_I| over real m
+    for i in range(6):
+        ms = numpy.linspace(-50, 50, 200001)
+        best = ms[numpy.argmin(numpy.abs(ms * r[i] + tstar[i]))]
+        assert abs(tgt[i] + inv[i] - best) < 1e-3
+    assert abs(w.sum() - 1.0) < 1e-12
+    # first Adam step moves every parameter by lr against its gradient sign
+    p = numpy.zeros(4)
+    gr = numpy.array([1.0, -2.0, 0.5, 0.0])
+    q, _, _ = mo.adam_step(p, gr, numpy.zeros(4), numpy.zeros(4), 1, 0.01)
+    assert numpy.allclose(q, [-0.01, 0.01, -0.01, 0.0], atol=1e-9)
