@@ -174,7 +174,14 @@ typedef struct fqehip_sector {
   const uint16_t* hbb_hi;
   const uint16_t* hbb_lo;
   int64_t ldh;
+  /* Tile shape of the complex64 row kernel when it also scatters beta
+   * excitations (hbb_hi NULL): FQEHIP_TILE_ROWS4 (default, 0 selects it)
+   * or FQEHIP_TILE_ROWS8 (4 or 8 waves per alpha row). */
+  int tile;
 } fqehip_sector;
+
+#define FQEHIP_TILE_ROWS4 4
+#define FQEHIP_TILE_ROWS8 8
 
 typedef struct fqehip_operator {
   int complex_op;   /* 0: A is (npair+1) x npair; 1: 2(npair+1) x 2npair */
@@ -346,6 +353,37 @@ int fqehip_mlp_loss_grad(const float* d_params, int64_t m, const float* d_x,
 int fqehip_mlp_adam(float* d_params, const float* d_grad, float* d_m,
                     float* d_v, int step, float lr, float beta1, float beta2,
                     float eps, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Batched sectors: one apply and a fixed-length Taylor series per sector,
+ * recorded once into a HIP graph over several capture streams and replayed
+ * with one launch.  The reference evolves sectors one by one from Python
+ * (src/fqe/wavefunction.py:509-611, :961-1054); its Taylor loop stops on
+ * convergence instead of after a fixed number of terms (:556-568).
+ * ------------------------------------------------------------------- */
+typedef struct fqehip_batch_item {
+  fqehip_sector sector;      /* copied; its tables must outlive the batch */
+  fqehip_operator op;        /* real or complex two-body operator */
+  const void* d_coeff0;      /* C0 (storage precision), read only */
+  void* d_apply_out;         /* H C0, or NULL to skip the apply */
+  void* d_term;              /* Taylor term (storage precision) */
+  double* d_result;          /* complex128 sum_{k<=nterms} (-i t H)^k C0 / k! */
+  void* d_workspace;         /* fqehip_apply_workspace_bytes(sector) */
+} fqehip_batch_item;
+
+typedef struct fqehip_batch fqehip_batch;
+
+/* Records the batch into a graph with up to nstreams parallel branches
+ * (1..32), on capture streams of the batch's own (`stream` is not used
+ * while recording).  Buffers are referenced, not copied: every launch on
+ * a stream reads d_coeff0 and rewrites the outputs. */
+int fqehip_batch_create(int precision, int nitems,
+                        const fqehip_batch_item* items, double time,
+                        int nterms, int nstreams, void* stream,
+                        fqehip_batch** out);
+int fqehip_batch_launch(fqehip_batch* batch, void* stream);
+int fqehip_batch_nodes(const fqehip_batch* batch);
+void fqehip_batch_destroy(fqehip_batch* batch);
 
 #ifdef __cplusplus
 }

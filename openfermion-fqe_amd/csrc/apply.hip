@@ -392,7 +392,7 @@ int launch_fused(int precision, int norb, int npair, int64_t lena, int64_t lenb,
                  const double* a, const int32_t* pta, const int32_t* ptb,
                  const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
                  int nslot, const void* coeff, void* slots, void* rowacc,
-                 bool split, hipStream_t st, const int32_t* done);
+                 bool split, int tile, hipStream_t st, const int32_t* done);
 // Defined in beta_block.hip.
 bool beta_gemm_shape_ok(int64_t lena, int64_t lenb, int64_t ldh);
 int launch_beta_gemm(int64_t lena, int64_t lenb, int64_t ldh, const void* hhi,
@@ -581,6 +581,8 @@ int check_sector_op(int precision, const fqehip_sector* s,
   FQEHIP_CHECK_ARG(op->h1 || op->a, "operator has neither A nor h1");
   FQEHIP_CHECK_ARG(op->complex_op == 0 || op->complex_op == 1,
                    "complex_op=%d", op->complex_op);
+  FQEHIP_CHECK_ARG(s->tile == 0 || s->tile == FQEHIP_TILE_ROWS4 ||
+                       s->tile == FQEHIP_TILE_ROWS8, "tile=%d", s->tile);
   return FQEHIP_OK;
 }
 
@@ -627,7 +629,8 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
   }
   return launch_fused(precision, s->norb, s->npair, s->lena, s->lenb, op->a,
                       s->ptab_a, s->ptab_b, s->fwd_a, s->fwd_b, s->slot_a,
-                      s->nslot, d_coeff, ws, rowacc, split, st, d_done);
+                      s->nslot, d_coeff, ws, rowacc, split, s->tile, st,
+                      d_done);
 }
 
 extern "C" int fqehip_apply_slot_reduce(int precision, const fqehip_sector* s,

@@ -22,7 +22,6 @@
 // slot rows (n_alpha (norb - n_alpha) per determinant), rowacc and out.
 #include "fqehip_common.h"
 
-#include <cstdlib>
 
 namespace fqehip {
 namespace {
@@ -681,16 +680,6 @@ size_t f64_lds_bytes(int64_t lenb) {
 
 constexpr size_t kLdsLimit = 160 * 1024;
 
-// Tile variant of the mixed-precision fused kernel: FQEHIP_BF16_WAVES=8
-// selects 8 waves per workgroup, anything else 4.
-int bf16_variant() {
-  static const int v = [] {
-    const char* e = getenv("FQEHIP_BF16_WAVES");
-    return (e && atoi(e) == 8) ? 8 : 4;
-  }();
-  return v;
-}
-
 unsigned fused_grid(int64_t lena) {
   return static_cast<unsigned>(lena < 65536 ? lena : 65536);
 }
@@ -719,7 +708,8 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
                  int64_t lenb, const double* a, const int32_t* pta,
                  const int32_t* ptb, const int32_t* fwa, const int32_t* fwb,
                  const int32_t* sla, int nslot, const void* coeff, void* slots,
-                 void* rowacc, bool split, hipStream_t st, const int32_t* done) {
+                 void* rowacc, bool split, int tile, hipStream_t st,
+                 const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
   if (split) {
@@ -745,7 +735,7 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
     const size_t lds = bf16_lds_bytes(lenb);
     // variant 4: 4 waves x 2 row tiles, two workgroups per CU;
     // variant 8: 8 waves x 1 row tile, one workgroup per CU
-    if (bf16_variant() == 8)
+    if (tile == FQEHIP_TILE_ROWS8)
       k_fused_bf16x3<8, 2, false><<<fused_grid(lena), 512, lds, st>>>(
           norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
           static_cast<const float*>(coeff), static_cast<float*>(slots),

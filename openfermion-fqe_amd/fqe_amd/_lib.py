@@ -38,11 +38,18 @@ class Sector(ctypes.Structure):
                 ("ptab_a", c_void_p), ("ptab_b", c_void_p),
                 ("fwd_a", c_void_p), ("fwd_b", c_void_p),
                 ("slot_a", c_void_p), ("nslot", c_int),
-                ("hbb_hi", c_void_p), ("hbb_lo", c_void_p), ("ldh", c_int64)]
+                ("hbb_hi", c_void_p), ("hbb_lo", c_void_p), ("ldh", c_int64),
+                ("tile", c_int)]
 
 
 class Operator(ctypes.Structure):
     _fields_ = [("complex_op", c_int), ("a", c_void_p), ("h1", c_void_p)]
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("sector", Sector), ("op", Operator), ("d_coeff0", c_void_p),
+                ("d_apply_out", c_void_p), ("d_term", c_void_p),
+                ("d_result", c_void_p), ("d_workspace", c_void_p)]
 
 
 _SIGNATURES = {
@@ -88,6 +95,12 @@ _SIGNATURES = {
                                          c_void_p]),
     "fqehip_apply_workspace_bytes": (c_int64, [c_int, POINTER(Sector)]),
     "fqehip_beta_block_ld": (c_int64, [c_int64]),
+    "fqehip_batch_create": (c_int, [c_int, c_int, POINTER(BatchItem), c_double,
+                                    c_int, c_int, c_void_p,
+                                    POINTER(c_void_p)]),
+    "fqehip_batch_launch": (c_int, [c_void_p, c_void_p]),
+    "fqehip_batch_nodes": (c_int, [c_void_p]),
+    "fqehip_batch_destroy": (None, [c_void_p]),
     "fqehip_diagonal": (c_int, [c_int, c_int64, c_int64, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p]),
@@ -169,7 +182,8 @@ def ptr(t) -> int:
 
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
-    if name.endswith(("_count", "_bytes", "_supported", "_ld", "_floats")):
+    if name.endswith(("_count", "_bytes", "_supported", "_ld", "_floats",
+                      "_nodes", "_destroy")):
         return rc
     if rc != OK:
         msg = lib().fqehip_last_error().decode(errors="replace")

@@ -193,7 +193,8 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
              dtype: torch.dtype = torch.complex128,
              eps: float = 1e-8, verbose: bool = False,
              diag: Optional[torch.Tensor] = None,
-             recorder: Optional[Callable] = None) -> DavidsonResult:
+             recorder: Optional[Callable] = None,
+             seed: int = 0) -> DavidsonResult:
     """Lowest eigenpair to ||H u - theta u|| < tol.
 
     ``correction(theta, u, r)`` may return an FP32 [n] device tensor added
@@ -202,7 +203,10 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
     (complex128: FP64; complex64: mixed precision, whose apply error bounds
     the reachable residual).  ``diag`` reuses a computed <I|H|I>;
     ``recorder(it, theta, u, r)`` sees every iteration's Ritz pair and
-    residual (training data of the learned preconditioner)."""
+    residual (training data of the learned preconditioner).  Without a
+    guess the start vector is the lowest-diagonal determinant plus 1e-3 of
+    a seeded random vector, so that no symmetry sector of the sector's
+    eigenvectors is missing from it."""
     norb = hamil.dim()
     graph = FciGraph(n_alpha, n_beta, norb)
     n = graph.lena() * graph.lenb()
@@ -216,7 +220,11 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
     if guess is not None:
         x.copy_(guess.reshape(-1).to(torch.complex128))
     else:
-        x[int(torch.argmin(diag).item())] = 1.0
+        rng = numpy.random.default_rng(seed)
+        noise = rng.normal(size=n) + 1j * rng.normal(size=n)
+        noise *= 1e-3 / numpy.linalg.norm(noise)
+        x.copy_(torch.from_numpy(noise))
+        x[int(torch.argmin(diag).item())] += 1.0
     sub.orthonormalize(x)
     sub.append(x, apply_fn)
     u = torch.empty_like(x)

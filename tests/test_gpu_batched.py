@@ -1,0 +1,103 @@
+"""GPU tests of the batched-sector graph (one apply + fixed 10-term Taylor
+per sector, fqehip_batch_*) against the oracle, the eager sequential path
+and across graph replays; every kernel variant of the autotuner against
+the oracle.  Replays agree to rounding, not bit for bit: the row kernels
+accumulate in LDS with atomic adds, whose order varies between runs."""
+
+import numpy
+import pytest
+
+from oracle import fqe_oracle as fo
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_TOL = 2e-4
+
+
+@pytest.fixture(scope="module")
+def fqe():
+    import torch
+    assert torch.cuda.is_available()
+    import fqe_amd
+    return fqe_amd
+
+
+def rel(a, b):
+    return numpy.linalg.norm(a - b) / max(numpy.linalg.norm(b), 1e-300)
+
+
+def _oracle(spec, t, nterms):
+    g = fo.OracleGraph(spec.nalpha, spec.nbeta, spec.norb)
+    c = spec.coeff.astype(numpy.complex128)
+    out = fo.apply_12(c, spec.h1, spec.h2, g)
+    term = c.copy()
+    res = c.copy()
+    for k in range(1, nterms + 1):
+        term = (-1j * t / k) * fo.apply_12(term, spec.h1, spec.h2, g)
+        res = res + term
+    return out, res
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_batched_graph_matches_oracle_and_sequential(fqe, dtype):
+    import torch
+    from fqe_amd import batched
+    tdt = torch.complex128 if dtype == "c128" else torch.complex64
+    tol = FP64_TOL if dtype == "c128" else MIXED_TOL
+    same = 1e-14 if dtype == "c128" else 1e-6    # run-to-run rounding
+    specs = batched.random_specs(14, seed=33, norb_range=(4, 8))
+    print("shapes", [(s.norb, s.nalpha, s.nbeta) for s in specs], flush=True)
+    bt = batched.BatchedTaylor(specs, 0.1, nterms=10, dtype=tdt, nstreams=4)
+    bt.build()
+    assert bt.nodes() > 0
+    refs = [_oracle(sp, 0.1, 10) for sp in specs]
+    for rep in range(3):
+        bt.launch()
+        got = bt.results()
+        for (ap, ev), (rap, rev), sp in zip(got, refs, specs):
+            assert rel(ap, rap) < tol, (rep, sp.norb, sp.nalpha, sp.nbeta)
+            assert rel(ev, rev) < tol, (rep, sp.norb, sp.nalpha, sp.nbeta)
+        if rep == 0:
+            first = got
+        else:
+            for (a0, e0), (a1, e1) in zip(first, got):
+                assert rel(a1, a0) < same and rel(e1, e0) < same
+        # eager work between replays
+        torch.zeros(1000, device="cuda").sum().item()
+    bt.run_sequential()
+    seq = bt.results()
+    for (a0, e0), (a1, e1) in zip(first, seq):
+        assert rel(a1, a0) < same and rel(e1, e0) < same
+    bt.close()
+
+
+def test_every_variant_matches_oracle(fqe):
+    import torch
+    from fqe_amd import batched
+    specs = batched.random_specs(3, seed=5, norb_range=(7, 8))
+    for sp in specs:
+        ref, _ = _oracle(sp, 0.1, 0)
+        base = batched._Prepared(sp, torch.complex64, None)
+        for var in batched.variants_for(base.op, base.prec):
+            bt = batched.BatchedTaylor([sp], 0.1, nterms=0,
+                                       dtype=torch.complex64,
+                                       tiles={f"{sp.norb},{sp.nalpha},{sp.nbeta}": var},
+                                       nstreams=1)
+            assert bt.sectors[0].variant == var
+            bt.build()
+            bt.launch()
+            ap, ev = bt.results()[0]
+            assert rel(ap, ref) < MIXED_TOL, var
+            assert rel(ev, sp.coeff) < 1e-7, var
+            bt.close()
+
+
+def test_autotune_picks_a_valid_variant(fqe):
+    import torch
+    from fqe_amd import batched
+    specs = batched.random_specs(6, seed=8, norb_range=(5, 9))
+    table = batched.autotune(specs, torch.complex64, reps=2)
+    for sp in specs:
+        key = f"{sp.norb},{sp.nalpha},{sp.nbeta}"
+        assert table[key] in batched.VARIANTS
