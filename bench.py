@@ -263,9 +263,26 @@ def kernel_work(op, graph, n, esz, mixed):
     }
 
 
-def roofline_entry(name, per_ms, work, mode):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_norb14_pmc_traffic.md")
+
+
+def measured_traffic(norb, na, nb):
+    """HBM bytes per launch of each kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (norb=14 7a7b only), else {}."""
+    if (norb, na, nb) != (14, 7, 7) or not os.path.exists(TRAFFIC_FILE):
+        return {}
+    text = open(TRAFFIC_FILE).read()
+    i = text.find("```json")
+    if i < 0:
+        return {}
+    j = text.find("```", i + 7)
+    return json.loads(text[i + 7:j])["traffic_bytes_per_launch"]
+
+
+def roofline_entry(name, per_ms, work, mode, traffic=None):
     """Roofline of one kernel: the resource whose time at peak is longer
-    bounds it (HBM bytes / 8 TB/s vs MFMA flops / peak)."""
+    bounds it (HBM bytes / 8 TB/s vs MFMA flops / peak).  ``traffic`` is
+    the measured HBM bytes per launch (rocprofv3 counters), if known."""
     t = per_ms[name] * 1e-3
     w = work[name]
     peak_fl = PEAK_TFLOPS["f64"] if mode == "fp64" else PEAK_TFLOPS["bf16"]
@@ -276,13 +293,38 @@ def roofline_entry(name, per_ms, work, mode):
         ach = mult * w["flops"] / t / 1e12
         return {"kernel": name, "bound": "mfma", "achieved": round(ach, 2),
                 "peak": peak_fl, "unit": "TFLOP/s",
-                "frac": round(ach / peak_fl, 4), "traffic": None,
-                "flops_per_launch": mult * w["flops"]}
+                "frac": round(ach / peak_fl, 4), "traffic": traffic,
+                "flops_per_launch": mult * w["flops"],
+                "algorithmic_bytes_per_launch": w["bytes"]}
     ach = w["bytes"] / t / 1e9
     return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
             "bytes_per_launch": w["bytes"]}
+
+
+def host_boundary_rate(coeff, h1, h2, norb, na, nb, reps=3):
+    """Mixed-precision applies/s when the caller hands over and takes back
+    host arrays (Wavefunction.set_wfn from numpy, apply, get_coeff): the
+    PCIe-inclusive rate of the Python boundary."""
+    import numpy
+    import torch
+    import fqe_amd as fqe
+    key = (na + nb, na - nb)
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]], dtype=numpy.complex64)
+    times = []
+    for r in range(reps + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        wfn.set_wfn(strategy="from_data", raw_data={key: coeff})
+        wfn.apply(ham).get_coeff(key)
+        torch.cuda.synchronize()
+        if r:
+            times.append(time.perf_counter() - t0)
+    del wfn, ham
+    torch.cuda.empty_cache()
+    return reps / sum(times)
 
 
 def cpu_baseline(coeff, h1, h2, norb, na, nb):
@@ -375,6 +417,7 @@ def main():
         return float(numpy.linalg.norm(a - b) / numpy.linalg.norm(b))
 
     err_mixed_fp64 = rel(outs["mixed"], outs["fp64"])
+    host_rate = host_boundary_rate(coeff, h1, h2, norb, na, nb)
 
     cpu, ref_out = (None, None)
     if not args.no_cpu_baseline:
@@ -388,6 +431,7 @@ def main():
     work_d = kernel_work(op, graph, n, 8, False)
     dom_m = max(mixed["per"], key=mixed["per"].get)
     dom_d = max(fp64["per"], key=fp64["per"].get)
+    traffic = measured_traffic(norb, na, nb)
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -410,12 +454,15 @@ def main():
         "fp64": {"value": round(fp64_value, 3), "unit": "applies/s",
                  "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3),
                  "kernels_ms": {k: round(v, 4) for k, v in fp64["per"].items()},
-                 "roofline": roofline_entry(dom_d, fp64["per"], work_d,
-                                            "fp64")},
+                 "roofline": roofline_entry(
+                     dom_d, fp64["per"], work_d, "fp64",
+                     traffic.get(dom_d + "_fp64"))},
         "speedup_mixed_vs_fp64": round(value / fp64_value, 3),
+        "host_boundary_applies_per_s": round(host_rate, 3),
         "rel_err_mixed_vs_fp64_gpu": err_mixed_fp64,
         "kernels_ms": {k: round(v, 4) for k, v in mixed["per"].items()},
-        "roofline": roofline_entry(dom_m, mixed["per"], work_m, "mixed"),
+        "roofline": roofline_entry(dom_m, mixed["per"], work_m, "mixed",
+                                   traffic.get(dom_m)),
         "cpu_baseline": cpu,
     }
     if ref_out is not None:

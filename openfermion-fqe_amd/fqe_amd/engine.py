@@ -218,3 +218,27 @@ def apply_sector(graph, coeff: torch.Tensor, op: PreparedOperator,
 
 def norm_partials_count(n: int) -> int:
     return int(_lib.lib().fqehip_norm_partials_count(n))
+
+
+_DOT_WS = {}
+
+
+def dot128(a: torch.Tensor, b: torch.Tensor) -> complex:
+    """<a|b> = sum conj(a) b over complex128 device vectors, FP64 with a
+    fixed summation order (fqehip_block_dot)."""
+    a = a.reshape(-1)
+    b = b.reshape(-1)
+    assert a.dtype == torch.complex128 and b.dtype == torch.complex128
+    assert a.is_contiguous() and b.is_contiguous() and a.numel() == b.numel()
+    dev = a.device
+    if dev.index not in _DOT_WS:
+        nbytes = int(_lib.call("fqehip_block_dot_workspace_bytes", 1))
+        _DOT_WS[dev.index] = (torch.empty(nbytes, dtype=torch.uint8,
+                                          device=dev),
+                              torch.empty(1, dtype=torch.complex128,
+                                          device=dev))
+    ws, out = _DOT_WS[dev.index]
+    _lib.call("fqehip_block_dot", a.numel(), 1, _lib.ptr(a), a.numel(),
+              _lib.ptr(b), _lib.ptr(out), _lib.ptr(ws), _lib.stream_ptr())
+    return complex(out.item())
+

@@ -214,9 +214,11 @@ class FqeData:
         return self
 
     def norm(self) -> float:
+        """FP64 2-norm on the device (fqehip_block_dot)."""
         c = self.device_coeff()
-        return float(torch.linalg.vector_norm(
-            c.to(torch.complex128) if c.dtype != torch.complex128 else c))
+        c = (c.to(torch.complex128) if c.dtype != torch.complex128
+             else c).contiguous()
+        return float(numpy.sqrt(max(engine.dot128(c, c).real, 0.0)))
 
     def scale(self, sval: complex) -> None:
         self._dev = self.device_coeff() * sval

@@ -50,10 +50,11 @@ def rand_wfn(adim: int, bdim: int) -> numpy.ndarray:
 def vdot(wfn1, wfn2) -> complex:
     """sum over common sectors of <wfn1|wfn2> (reference
     ``src/fqe/util.py:506-530``), on the device in complex128."""
+    from . import engine
     keys = [k for k in wfn1.sectors() if k in wfn2.sectors()]
     total = 0.0 + 0.0j
     for key in keys:
-        a = wfn1.sector(key).device_coeff().to(torch.complex128).reshape(-1)
-        b = wfn2.sector(key).device_coeff().to(torch.complex128).reshape(-1)
-        total += complex(torch.vdot(a, b))
+        a = wfn1.sector(key).device_coeff().to(torch.complex128).contiguous()
+        b = wfn2.sector(key).device_coeff().to(torch.complex128).contiguous()
+        total += engine.dot128(a, b)
     return total

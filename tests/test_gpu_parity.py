@@ -499,3 +499,49 @@ def test_split_apply(fqe, na, nb, norb):
     print(f"split rel err {err:.3e} vs rows {rel(rows, ref):.3e}", flush=True)
     assert err < MIXED_TOL
     assert rel(split, rows) < MIXED_TOL
+
+
+# --------------------------------------------------------------------------
+# Chebyshev propagator (reference src/fqe/wavefunction.py:570-602; the
+# reference's tests tests/evolution_test.py:777-808 and
+# tests/wavefunction_test.py:649-676 compare it with the Taylor series to
+# 1e-8; here also with the exact exponential of the dense oracle matrix)
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("na,nb,norb", [(2, 1, 4), (2, 2, 5)])
+def test_chebyshev_matches_taylor_and_expm(fqe, na, nb, norb):
+    from scipy.linalg import expm
+    rng = numpy.random.default_rng(40 + norb)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    dense = fo.dense_matrix(h1, h2, g)
+    w = numpy.linalg.eigvalsh(dense)
+    exact = (expm(-1j * 0.1 * dense) @ c.reshape(-1)).reshape(c.shape)
+    key = (na + nb, na - nb)
+    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]])
+    wfn.set_wfn(strategy="from_data", raw_data={key: c})
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    cheb = wfn.apply_generated_unitary(0.1, "chebyshev", ham, accuracy=1e-14,
+                                       spec_lim=(w[0], w[-1]))
+    tay = wfn.apply_generated_unitary(0.1, "taylor", ham)
+    assert rel(cheb.get_coeff(key), exact) < 1e-11
+    assert numpy.linalg.norm(cheb.get_coeff(key) - tay.get_coeff(key)) < 1e-8
+    with pytest.raises(RuntimeError):
+        wfn.apply_generated_unitary(0.1, "chebyshev", ham, expansion=3,
+                                    spec_lim=(w[0], w[-1]))
+
+
+def test_norm_and_vdot_fp64_kernels(fqe):
+    rng = numpy.random.default_rng(77)
+    norb, na, nb = 6, 3, 2
+    g = fo.OracleGraph(na, nb, norb)
+    a = fo.synthetic_coeff(g.lena, g.lenb, rng) * 3.0
+    b = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    key = (na + nb, na - nb)
+    wa = fqe.Wavefunction([[na + nb, na - nb, norb]])
+    wb = fqe.Wavefunction([[na + nb, na - nb, norb]])
+    wa.set_wfn(strategy="from_data", raw_data={key: a})
+    wb.set_wfn(strategy="from_data", raw_data={key: b})
+    assert abs(wa.norm() - numpy.linalg.norm(a)) < 1e-13
+    assert abs(fqe.vdot(wa, wb) - numpy.vdot(a, b)) < 1e-13
