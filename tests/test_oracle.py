@@ -316,3 +316,20 @@ def test_mlp_oracle_targets_and_adam():
     gr = numpy.array([1.0, -2.0, 0.5, 0.0])
     q, _, _ = mo.adam_step(p, gr, numpy.zeros(4), numpy.zeros(4), 1, 0.01)
     assert numpy.allclose(q, [-0.01, 0.01, -0.01, 0.0], atol=1e-9)
+
+
+def test_lih_fixture_energies():
+    """FQE's LiH data: <psi|H|psi> is the FCI energy of
+    tests/davidson_test.py:27, the lowest dense eigenvalue equals it, and
+    the Slater-Condon diagonal at the HF determinant is the HF energy of
+    tests/wavefunction_test.py:537-548."""
+    z = numpy.load(os.path.join(os.path.dirname(__file__), "golden",
+                                "lih_energy.npz"))
+    h1, h2, wfn = z["h1e"], z["h2e"], z["wfn"]
+    g = fo.OracleGraph(2, 2, 6)
+    e = numpy.vdot(wfn, fo.apply_12(wfn, h1, h2, g)) / numpy.vdot(wfn, wfn)
+    assert abs(e - (-8.877719570384043)) < 1e-10
+    w = numpy.linalg.eigvalsh(fo.dense_matrix(h1, h2, g))
+    assert abs(w[0] - (-8.877719570384043)) < 1e-10
+    d = fo.slater_condon_diagonal(h1, h2, g)
+    assert abs(d[0, 0] - (-8.857341498221992)) < 1e-12
