@@ -717,7 +717,9 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
       set_error("the split apply needs complex64 and npair + 1 <= %d", kBfKPad);
       return FQEHIP_ERR_INVALID;
     }
-    k_fused_bf16x3<4, 2, true><<<fused_grid(lena), 256, bf16_lds_bytes(lenb, true), st>>>(
+    // three workgroups per CU: the split kernel needs no LDS accumulator
+    // row, and the higher occupancy outweighs a few spilled registers
+    k_fused_bf16x3<4, 3, true><<<fused_grid(lena), 256, bf16_lds_bytes(lenb, true), st>>>(
         norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
         static_cast<const float*>(coeff), static_cast<float*>(slots),
         static_cast<float*>(rowacc), done);
