@@ -242,3 +242,18 @@ def dot128(a: torch.Tensor, b: torch.Tensor) -> complex:
               _lib.ptr(b), _lib.ptr(out), _lib.ptr(ws), _lib.stream_ptr())
     return complex(out.item())
 
+
+def block_dots(rows: torch.Tensor, x: torch.Tensor) -> numpy.ndarray:
+    """[sum_i conj(rows[j, i]) x[i] for j] over complex128 device data
+    (fqehip_block_dot, FP64, fixed summation order)."""
+    k, n = rows.shape
+    assert rows.dtype == torch.complex128 and x.dtype == torch.complex128
+    assert rows.is_contiguous() and x.is_contiguous() and x.numel() == n
+    ws = torch.empty(max(int(_lib.call("fqehip_block_dot_workspace_bytes",
+                                       k)), 16),
+                     dtype=torch.uint8, device=rows.device)
+    out = torch.empty(k, dtype=torch.complex128, device=rows.device)
+    _lib.call("fqehip_block_dot", n, k, _lib.ptr(rows), n, _lib.ptr(x),
+              _lib.ptr(out), _lib.ptr(ws), _lib.stream_ptr())
+    return out.cpu().numpy()
+

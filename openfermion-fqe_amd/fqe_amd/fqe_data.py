@@ -206,6 +206,46 @@ class FqeData:
                                                                self.lenb())
         return d.reshape(shape)
 
+    # -- reduced density matrices ------------------------------------------
+    def _dvec128(self) -> torch.Tensor:
+        """[norb^2, lena*lenb] complex128 device dvec (ordered pairs)."""
+        d = self.calculate_dvec_spatial_device(compressed=False)
+        n2 = self.norb() * self.norb()
+        return d.reshape(n2, -1).to(torch.complex128).contiguous()
+
+    def _coeff128(self) -> torch.Tensor:
+        return self.device_coeff().reshape(-1).to(torch.complex128).contiguous()
+
+    def rdm1(self, bradata: Optional['FqeData'] = None) -> Tuple[numpy.ndarray]:
+        """1-RDM rdm1[p, q] = <bra|E_pq|ket> (reference
+        ``src/fqe/fqe_data.py:1668-1720``): one dvec gather of the bra and
+        norb^2 FP64 inner products with the ket on the device."""
+        bra = self if bradata is None else bradata
+        norb = self.norb()
+        t = engine.block_dots(bra._dvec128(), self._coeff128())
+        return (t.reshape(norb, norb).T.copy(),)
+
+    def rdm12(self, bradata: Optional['FqeData'] = None
+              ) -> Tuple[numpy.ndarray, numpy.ndarray]:
+        """1- and 2-RDMs in the reference's convention
+        (``src/fqe/fqe_data.py:1767-1782``):
+        rdm2[j, k, i, l] = -<E_ij bra|E_kl ket> + delta_ki rdm1[j, l].
+        The norb^2 x norb^2 overlap matrix of the two dvecs is formed with
+        the FP64 block inner-product kernel, one column per ket pair."""
+        bra = self if bradata is None else bradata
+        norb = self.norb()
+        n2 = norb * norb
+        d = self._dvec128()
+        d2 = d if bradata is None else bra._dvec128()
+        out1 = engine.block_dots(d2, self._coeff128()).reshape(norb, norb).T
+        m = numpy.empty((n2, n2), dtype=numpy.complex128)
+        for q in range(n2):
+            m[:, q] = engine.block_dots(d2, d[q].contiguous())
+        out2 = -m.reshape((norb,) * 4).transpose(1, 2, 0, 3).copy()
+        for i in range(norb):
+            out2[:, i, i, :] += out1
+        return out1.copy(), out2
+
     def ax_plus_y(self, sval: complex, other: 'FqeData') -> 'FqeData':
         assert hash(self) == hash(other)
         self._dev = self.device_coeff() + other.device_coeff().to(

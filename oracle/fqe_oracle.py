@@ -403,3 +403,18 @@ def synthetic_workload(norb: int, nalpha: int, nbeta: int, seed: int,
     coeff = synthetic_coeff(lena, lenb, rng)
     h1, h2 = synthetic_hamiltonian(norb, nfactors, rng)
     return coeff, h1, h2
+
+def rdm12(coeff: numpy.ndarray, g: OracleGraph, bra=None):
+    """1- and 2-RDMs from the dvec (reference ``src/fqe/fqe_data.py:
+    1767-1782``, its non-accelerated branch): rdm1 = (sum_I conj(D2) C)^T
+    and rdm2[j, k, i, l] = -sum_I conj(D2[i, j]) D[k, l] + delta_ki
+    rdm1[j, l], with D = dvec(ket), D2 = dvec(bra)."""
+    d = dvec(coeff, g)
+    d2 = d if bra is None else dvec(bra, g)
+    out1 = numpy.tensordot(d2.conj(), coeff, axes=((2, 3), (0, 1))).T
+    m = numpy.tensordot(d2.conj(), d, axes=((2, 3), (2, 3)))
+    out2 = -m.transpose(1, 2, 0, 3)
+    for i in range(g.norb):
+        out2[:, i, i, :] += out1
+    return out1, out2
+

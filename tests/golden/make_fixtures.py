@@ -7,7 +7,7 @@ test data.  Nothing of the reference is imported or executed:
   ``numpy.fromfile`` as ``tests/unittest_data/fqe_data_loader.py:28-31``
   does).  Copied as arrays: the FqeData(2,3,6), (2,1,4) and (1,1,2)
   coefficients, h1, h2 and the FQE outputs of apply((h1,)), apply((0,h2))
-  and apply((h1,h2)).
+  and apply((h1,h2)); and the 1- and 2-RDMs of FqeData(2,1,4).
 * literal tables are read from the reference test sources with
   ``ast.literal_eval`` on their literal nodes: strings and addresses of
   ``tests/fci_graph_test.py:33-112``, the n=4 maps of ``:151-190``, and the
@@ -47,6 +47,13 @@ def fqe_data_arrays():
                               ).reshape(shape)
         arrays[f"h1_{tag}"] = raw(f"h1{tag}.npy").reshape(n, n)
         arrays[f"h2_{tag}"] = raw(f"h2{tag}.npy").reshape(n, n, n, n)
+        if tag == "020104":
+            # FqeData(2,1,4).rdm1() / rdm12() outputs (generate_data.py:114-123)
+            for order in (1, 2):
+                shape_r = (n,) * (2 * order)
+                arrays[f"rdm{order}_{tag}"] = (
+                    raw(f"dr{order}{tag}.npy") + 1j * raw(f"di{order}{tag}.npy")
+                ).reshape(shape_r)
         for suffix in ("1", "2", "12"):
             arrays[f"out{suffix}_{tag}"] = (
                 raw(f"cr{tag}_{suffix}.npy") + 1j * raw(f"ci{tag}_{suffix}.npy")

@@ -545,3 +545,44 @@ def test_norm_and_vdot_fp64_kernels(fqe):
     wb.set_wfn(strategy="from_data", raw_data={key: b})
     assert abs(wa.norm() - numpy.linalg.norm(a)) < 1e-13
     assert abs(fqe.vdot(wa, wb) - numpy.vdot(a, b)) < 1e-13
+
+
+# --------------------------------------------------------------------------
+# reduced density matrices (reference src/fqe/fqe_data.py:1668-1782;
+# FQE's own outputs tests/unittest_data/fqe_data/{dr,di}{1,2}020104.npy,
+# tests/fqe_data_test.py:669-700)
+# --------------------------------------------------------------------------
+
+def test_rdm12_match_fqe_golden(fqe):
+    z = numpy.load(os.path.join(GOLDEN, "fqe_data_golden.npz"))
+    d = fqe.FqeData(2, 1, 4)
+    d.coeff = z["c_020104"]
+    (r1,) = d.rdm1()
+    assert numpy.max(numpy.abs(r1 - z["rdm1_020104"])) < 1e-13
+    o1, o2 = d.rdm12()
+    assert numpy.max(numpy.abs(o1 - z["rdm1_020104"])) < 1e-13
+    assert numpy.max(numpy.abs(o2 - z["rdm2_020104"])) < 1e-13
+
+
+def test_rdm1_contracts_to_the_one_body_energy(fqe):
+    """sum_pq h1[p,q] rdm1[p,q] = <C|H1|C> for a random one-body h1, and
+    the transition RDM with a second state matches <bra|H1|ket>."""
+    rng = numpy.random.default_rng(91)
+    norb, na, nb = 6, 3, 2
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    b = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    h1 = rng.normal(size=(norb, norb)) + 1j * rng.normal(size=(norb, norb))
+    ket = fqe.FqeData(na, nb, norb)
+    ket.coeff = c
+    bra = fqe.FqeData(na, nb, norb)
+    bra.coeff = b
+    (r1,) = ket.rdm1()
+    assert abs(numpy.sum(h1 * r1) - numpy.vdot(c, fo.apply_1(c, h1, g))) < 1e-12
+    (t1,) = ket.rdm1(bra)
+    assert abs(numpy.sum(h1 * t1) - numpy.vdot(b, fo.apply_1(c, h1, g))) < 1e-12
+    # two-body: <C|sum h2[i,j,k,l] a+_i a+_j a_k a_l|C> from rdm2
+    h2 = rng.normal(size=(norb,) * 4)
+    _, r2 = ket.rdm12()
+    e2 = numpy.vdot(c, fo.apply_12(c, numpy.zeros((norb, norb)), h2, g))
+    assert abs(numpy.einsum("ijkl,ijkl->", h2, r2) - e2) < 1e-11

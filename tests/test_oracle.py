@@ -333,3 +333,19 @@ def test_lih_fixture_energies():
     assert abs(w[0] - (-8.877719570384043)) < 1e-10
     d = fo.slater_condon_diagonal(h1, h2, g)
     assert abs(d[0, 0] - (-8.857341498221992)) < 1e-12
+
+
+def test_rdm12_oracle_matches_fqe_golden_and_energy(golden):
+    """The oracle RDMs reproduce FQE's stored rdm1/rdm12 of FqeData(2,1,4)
+    (tests/fqe_data_test.py:669-700) and contract with h1 / h2 to the
+    energy of the apply."""
+    g = fo.OracleGraph(2, 1, 4)
+    c = golden["c_020104"]
+    r1, r2 = fo.rdm12(c, g)
+    assert numpy.max(numpy.abs(r1 - golden["rdm1_020104"])) < 1e-13
+    assert numpy.max(numpy.abs(r2 - golden["rdm2_020104"])) < 1e-13
+    rng = numpy.random.default_rng(4)
+    h2 = rng.normal(size=(4,) * 4)
+    h1 = rng.normal(size=(4, 4))
+    e = numpy.vdot(c, fo.apply_12(c, h1, h2, g))
+    assert abs(numpy.sum(h1 * r1) + numpy.einsum("ijkl,ijkl->", h2, r2) - e) < 1e-12
