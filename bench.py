@@ -3,7 +3,7 @@
 Workload (the headline configuration): norb=14, 7 alpha + 7 beta electrons
 (3,432 x 3,432 = 11,778,624 determinants), complex coefficients with real
 and imaginary parts i.i.d. N(0,1) from seed 2, normalised; h1/h2 from 14
-Cholesky-like factors (oracle.fqe_oracle.synthetic_workload).  A step is
+Cholesky-like factors (fqe_amd.synthetic.workload).  A step is
 one apply H|C>.
 
 The engine runs it twice: FP64 (complex128 storage, FP64 MFMA GEMM) and
@@ -59,7 +59,78 @@ def parse():
                          "sectors, norb 6..12, seed 3, one apply + 10-term "
                          "Taylor each) and print its own JSON line")
     ap.add_argument("--sectors", type=int, default=256)
+    ap.add_argument("--davidson", action="store_true",
+                    help="run the Davidson workload instead: train the "
+                         "learned preconditioner on small synthetic "
+                         "Hamiltonians, then solve held-out norb=14 7a7b "
+                         "ones (seed 4) to a 1e-8 residual with and without "
+                         "it; prints its own JSON line")
+    ap.add_argument("--train", type=int, default=120,
+                    help="training Hamiltonians (norb 8 and 9)")
+    ap.add_argument("--heldout", type=int, default=3,
+                    help="held-out norb=14 Hamiltonians")
     return ap.parse_args()
+
+
+def davidson_main(args):
+    """Davidson to a 1e-8 residual (FP64 applies, FP64 Gram-Schmidt and
+    Rayleigh-Ritz) with the plain diagonal preconditioner and with the
+    learned correction, on held-out norb=14 7a7b Hamiltonians."""
+    import numpy
+    import torch
+    import fqe_amd as fqe
+    from fqe_amd import synthetic
+    from fqe_amd.algorithm import mlp_precond as mp
+    t0 = time.perf_counter()
+    xs, ts, ws, its = [], [], [], []
+    rng = numpy.random.default_rng(40)
+    for j in range(args.train):
+        norb = 8 if j % 2 == 0 else 9
+        na, nb = norb // 2, norb - norb // 2
+        h1, h2 = synthetic.hamiltonian(norb, norb, rng)
+        ham = fqe.get_restricted_hamiltonian((h1, h2))
+        out = mp.correction_samples(ham, na, nb, per_snapshot=256, seed=j)
+        if out is not None:
+            xs.append(out[0]); ts.append(out[1]); ws.append(out[2])
+            its.append(out[3])
+    t_data = time.perf_counter() - t0
+    model = mp.MLPPreconditioner(seed=0)
+    x, t, w = torch.cat(xs), torch.cat(ts), torch.cat(ws)
+    t1 = time.perf_counter()
+    losses = model.train(x, t, w, epochs=20, batch=4096, lr=1e-3)
+    t_train = time.perf_counter() - t1
+    rng = numpy.random.default_rng(4)
+    plain, learned, tp, tl, gap = [], [], 0.0, 0.0, 0.0
+    for j in range(args.heldout):
+        h1, h2 = synthetic.hamiltonian(14, 14, rng)
+        ham = fqe.get_restricted_hamiltonian((h1, h2))
+        r = mp.solve_both(ham, 7, 7, model, tol=1e-8, max_iter=600)
+        plain.append(r["plain"].iterations)
+        learned.append(r["learned"].iterations)
+        tp += r["t_plain"]
+        tl += r["t_learned"]
+        gap = max(gap, abs(r["plain"].eigenvalues[0] -
+                           r["learned"].eigenvalues[0]))
+        print(f"held-out {j}: plain {plain[-1]} its {r['t_plain']:.1f} s, "
+              f"learned {learned[-1]} its {r['t_learned']:.1f} s",
+              file=sys.stderr, flush=True)
+    print(json.dumps({
+        "metric": "Davidson iterations to a 1e-8 residual, norb=14 7a7b "
+                  "(11,778,624 determinants), learned vs plain diagonal "
+                  "preconditioner",
+        "plain_iterations": plain, "learned_iterations": learned,
+        "mean_plain": float(numpy.mean(plain)),
+        "mean_learned": float(numpy.mean(learned)),
+        "iteration_reduction": float(1.0 - numpy.mean(learned) /
+                                     numpy.mean(plain)),
+        "seconds_plain": round(tp, 2), "seconds_learned": round(tl, 2),
+        "max_eigenvalue_gap": gap,
+        "train": {"hamiltonians": len(xs), "samples": int(x.shape[0]),
+                  "mean_plain_iterations": float(numpy.mean(its)),
+                  "loss_first_last": [losses[0], losses[-1]],
+                  "data_s": round(t_data, 1), "train_s": round(t_train, 1)},
+        "applies": "FP64 engine", "n_gpus": 1, "data": "synthetic",
+    }))
 
 
 def batched_main(args):
@@ -363,6 +434,9 @@ def main():
     if args.batched and args.gpus == 1:
         batched_main(args)
         return
+    if args.davidson and args.gpus == 1:
+        davidson_main(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not "
               f"supported in this build (multi-GPU sharding is designed, not "
@@ -372,7 +446,7 @@ def main():
     import torch
     import fqe_amd as fqe
     from fqe_amd import engine, _lib
-    from oracle import fqe_oracle as fo
+    from fqe_amd import synthetic
 
     if args.copy_probe:
         src = torch.empty(args.copy_probe, dtype=torch.uint8, device="cuda")
@@ -383,7 +457,7 @@ def main():
               file=sys.stderr, flush=True)
         del src, dst
     norb, na, nb = args.norb, args.nalpha, args.nbeta
-    coeff, h1, h2 = fo.synthetic_workload(norb, na, nb, args.seed, norb)
+    coeff, h1, h2 = synthetic.workload(norb, na, nb, args.seed, norb)
     graph = fqe.FciGraph(na, nb, norb)
     dev = _lib.device()
     op = engine.prepare_operator((h1, h2), norb, dev)

@@ -139,6 +139,28 @@ class Subspace:
         w, s = numpy.linalg.eigh(self.g[:self.k, :self.k])
         return w[:nroots], s[:, :nroots]
 
+    def restart(self, coef: numpy.ndarray) -> None:
+        """Replace the basis by V coef (coef: k x m with orthonormal
+        columns); the images follow as AV coef and G as coef^H G coef, so
+        no apply is needed."""
+        k, m = coef.shape
+        assert k == self.k and m <= self.maxk
+        dev = self.v.device
+        if getattr(self, "_tmp", None) is None or self._tmp.shape[1] < m:
+            self._tmp = torch.empty((2, m, self.n), dtype=torch.complex128,
+                                    device=dev)
+        tmp = self._tmp
+        for j in range(m):
+            c = numpy.ascontiguousarray(coef[:, j])
+            self.combine(c, tmp[0, j])
+            self.combine(c, tmp[1, j], rows=self.av)
+        self.v[:m].copy_(tmp[0, :m])
+        self.av[:m].copy_(tmp[1, :m])
+        g = coef.conj().T @ self.g[:k, :k] @ coef
+        self.g[:] = 0.0
+        self.g[:m, :m] = 0.5 * (g + g.conj().T)
+        self.k = m
+
 
 class _Applier:
     """H x for device vectors of one sector (engine apply, e_0 added)."""
@@ -194,7 +216,7 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
              eps: float = 1e-8, verbose: bool = False,
              diag: Optional[torch.Tensor] = None,
              recorder: Optional[Callable] = None,
-             seed: int = 0) -> DavidsonResult:
+             seed: int = 0, keep: int = 8) -> DavidsonResult:
     """Lowest eigenpair to ||H u - theta u|| < tol.
 
     ``correction(theta, u, r)`` may return an FP32 [n] device tensor added
@@ -203,7 +225,10 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
     (complex128: FP64; complex64: mixed precision, whose apply error bounds
     the reachable residual).  ``diag`` reuses a computed <I|H|I>;
     ``recorder(it, theta, u, r)`` sees every iteration's Ritz pair and
-    residual (training data of the learned preconditioner).  Without a
+    residual (training data of the learned preconditioner).  A full
+    subspace restarts thickly: the ``keep`` lowest Ritz vectors plus the
+    previous iteration's Ritz vector (orthogonalised), images and projected
+    matrix carried over without applies.  Without a
     guess the start vector is the lowest-diagonal determinant plus 1e-3 of
     a seeded random vector, so that no symmetry sector of the sector's
     eigenvectors is missing from it."""
@@ -234,6 +259,8 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
     theta = 0.0
     converged = False
     it = 0
+    prev = None                      # previous Ritz coefficients
+    keep = max(1, min(keep, maxk - 2))
     for it in range(1, max_iter + 1):
         w, s = sub.ritz(1)
         theta = float(w[0])
@@ -256,12 +283,23 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
                   _lib.ptr(diag), theta, eps, _lib.ptr(corr), _lib.ptr(x),
                   _lib.stream_ptr())
         if sub.k == maxk:
-            # restart on the current Ritz vector (its image is known)
-            sub.v[0].copy_(u)
-            sub.av[0].copy_(au)
-            sub.g[:] = 0.0
-            sub.g[0, 0] = theta
-            sub.k = 1
+            _, allv = numpy.linalg.eigh(sub.g[:sub.k, :sub.k])
+            cols = [allv[:, j] for j in range(keep)]
+            if prev is not None:
+                pv = numpy.zeros(sub.k, dtype=numpy.complex128)
+                pv[:len(prev)] = prev
+                for _ in range(2):
+                    for c in cols:
+                        pv -= c * numpy.vdot(c, pv)
+                nrm_p = numpy.linalg.norm(pv)
+                # a nearly dependent previous vector would only add
+                # rounding noise to the basis (a residual floor)
+                if nrm_p > 1e-3:
+                    cols.append(pv / nrm_p)
+            sub.restart(numpy.stack(cols, axis=1))
+            prev = None
+        else:
+            prev = coef.copy()
         nrm = sub.orthonormalize(x)
         if nrm < 1e-14:
             # the correction lies in the subspace: perturb with the residual

@@ -136,3 +136,21 @@ def test_davidsonliu_errors(fqe):
         dv.davidsonliu_fqe(ham, 1, [w1, w2], 4, 0, 6)
     ok = copy.deepcopy(w1)
     ok.normalize()
+
+
+def test_davidson_thick_restart_matches_dense_eigh(fqe):
+    """A subspace of 6 vectors forces many thick restarts (3 kept Ritz
+    vectors + the previous one); the solver still reaches the exact lowest
+    eigenvalue."""
+    from fqe_amd.algorithm import davidson as dv
+    rng = numpy.random.default_rng(23)
+    norb, na, nb = 7, 3, 3
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    w = numpy.linalg.eigvalsh(fo.dense_matrix(h1, h2, g))
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    res = dv.davidson(ham, na, nb, tol=1e-9, max_subspace=6, keep=3,
+                      max_iter=800)
+    print(f"thick restart: {res.iterations} iterations", flush=True)
+    assert res.converged
+    assert abs(res.eigenvalues[0] - w[0]) < 1e-10

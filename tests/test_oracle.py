@@ -349,3 +349,17 @@ def test_rdm12_oracle_matches_fqe_golden_and_energy(golden):
     h1 = rng.normal(size=(4, 4))
     e = numpy.vdot(c, fo.apply_12(c, h1, h2, g))
     assert abs(numpy.sum(h1 * r1) + numpy.einsum("ijkl,ijkl->", h2, r2) - e) < 1e-12
+
+
+@pytest.mark.parametrize("norb,na,nb,seed,k", [(8, 4, 4, 0, 8), (6, 2, 3, 5, 3)])
+def test_product_synthetic_inputs_equal_oracle_convention(norb, na, nb, seed, k):
+    """The benchmark's input generator (fqe_amd.synthetic) draws exactly the
+    arrays of the oracle's documented convention."""
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..",
+                                    "openfermion-fqe_amd"))
+    from fqe_amd import synthetic
+    got = synthetic.workload(norb, na, nb, seed, k)
+    ref = fo.synthetic_workload(norb, na, nb, seed, k)
+    for a, b in zip(got, ref):
+        assert numpy.allclose(a, b, rtol=0, atol=1e-15)
