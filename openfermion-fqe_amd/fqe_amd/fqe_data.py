@@ -73,12 +73,18 @@ class FqeData:
     def device_coeff(self) -> torch.Tensor:
         """The coefficients as a device tensor of the sector's dtype."""
         if self._dev is None:
-            host = numpy.ascontiguousarray(self._host, dtype=self._dtype)
+            host = numpy.asarray(self._host)
             if host.shape != (self.lena(), self.lenb()):
                 raise ValueError(
                     f"coefficient shape {host.shape} does not match the "
                     f"sector ({self.lena()}, {self.lenb()})")
-            self._dev = torch.from_numpy(host).to(_lib.device())
+            if not numpy.iscomplexobj(host):
+                host = host.astype(self._dtype)
+            # upload as given and convert on the device (a host-side
+            # complex128 -> complex64 pass costs more than the wider copy)
+            dev = torch.from_numpy(numpy.ascontiguousarray(host)).to(
+                _lib.device())
+            self._dev = dev.to(_TORCH_DTYPE[self._dtype]).contiguous()
             self._host = None
         return self._dev
 
