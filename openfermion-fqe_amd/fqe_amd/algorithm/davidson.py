@@ -203,6 +203,7 @@ class DavidsonResult:
     residuals: List[float] = field(default_factory=list)
     converged: bool = False
     seconds: float = 0.0
+    switch_iteration: int = 0        # mixed -> FP64 (dtype="mixed")
 
 
 Correction = Callable[[float, torch.Tensor, torch.Tensor], Optional[torch.Tensor]]
@@ -216,14 +217,20 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
              eps: float = 1e-8, verbose: bool = False,
              diag: Optional[torch.Tensor] = None,
              recorder: Optional[Callable] = None,
-             seed: int = 0, keep: int = 8) -> DavidsonResult:
+             seed: int = 0, keep: int = 8,
+             mixed_tol: float = 2e-5) -> DavidsonResult:
     """Lowest eigenpair to ||H u - theta u|| < tol.
 
     ``correction(theta, u, r)`` may return an FP32 [n] device tensor added
     to 1/(H_dd - theta) in the preconditioner; None keeps the plain
     diagonal preconditioner.  ``dtype`` selects the engine of the applies
     (complex128: FP64; complex64: mixed precision, whose apply error bounds
-    the reachable residual).  ``diag`` reuses a computed <I|H|I>;
+    the reachable residual; "mixed": mixed-precision applies until the
+    residual falls below ``mixed_tol`` * max(1, |theta|), or stalls within
+    50 times that (the mixed apply's error, about 4e-6 of ||H u||, is the
+    floor), then a
+    thick restart whose
+    kept vectors get FP64 images, and FP64 applies to the end).  ``diag`` reuses a computed <I|H|I>;
     ``recorder(it, theta, u, r)`` sees every iteration's Ritz pair and
     residual (training data of the learned preconditioner).  A full
     subspace restarts thickly: the ``keep`` lowest Ritz vectors plus the
@@ -238,7 +245,11 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
     dev = _lib.device()
     t0 = time.perf_counter()
     diag = (diagonal(hamil, graph) if diag is None else diag).reshape(-1)
-    apply_fn = _Applier(hamil, graph, dtype)
+    two_phase = dtype == "mixed"
+    apply_fn = _Applier(hamil, graph,
+                        torch.complex64 if two_phase else dtype)
+    applies = 0
+    switch_it = 0
     maxk = max(2, min(max_subspace, n))
     sub = Subspace(n, maxk, dev)
     x = torch.zeros(n, dtype=torch.complex128, device=dev)
@@ -275,6 +286,27 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
             print(f"davidson it {it} theta {theta:.12f} |r| {rn:.3e}")
         if recorder is not None:
             recorder(it, theta, u, r)
+        floor = mixed_tol * max(1.0, abs(theta))
+        if two_phase and (rn < floor or
+                          (rn < 50.0 * floor and len(res) > 30 and
+                           min(res[-15:]) > 0.5 * min(res[-30:-15]))):
+            # leave the mixed phase: thick restart on the lowest Ritz
+            # vectors with their images recomputed by the FP64 engine
+            two_phase = False
+            switch_it = it
+            applies += apply_fn.napply
+            apply_fn = _Applier(hamil, graph, torch.complex128)
+            _, allv = numpy.linalg.eigh(sub.g[:sub.k, :sub.k])
+            m = min(keep, sub.k)
+            sub.restart(allv[:, :m])
+            for j in range(m):
+                apply_fn(sub.v[j], sub.av[j])
+            col = numpy.stack([sub.dots(sub.av[j], k=m).cpu().numpy()
+                               for j in range(m)], axis=1)
+            sub.g[:] = 0.0
+            sub.g[:m, :m] = 0.5 * (col + col.conj().T)
+            prev = None
+            continue
         if rn < tol or sub.k >= n:
             converged = rn < tol
             break
@@ -308,8 +340,8 @@ def davidson(hamil: RestrictedHamiltonian, n_alpha: int, n_beta: int,
         sub.append(x, apply_fn)
     torch.cuda.synchronize()
     return DavidsonResult(numpy.array([theta]), u.reshape(1, -1).clone(), it,
-                          apply_fn.napply, res, converged,
-                          time.perf_counter() - t0)
+                          applies + apply_fn.napply, res, converged,
+                          time.perf_counter() - t0, switch_it)
 
 
 # ---------------------------------------------------------------------------

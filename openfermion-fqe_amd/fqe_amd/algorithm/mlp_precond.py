@@ -236,3 +236,11 @@ def solve_both(hamil, n_alpha: int, n_beta: int,
     t2 = time.perf_counter()
     return {"plain": plain, "learned": learned, "t_plain": t1 - t0,
             "t_learned": t2 - t1}
+
+
+def solve_mixed(hamil, n_alpha: int, n_beta: int, tol: float = 1e-8,
+                max_iter: int = 300, diag: Optional[torch.Tensor] = None):
+    """Plain-preconditioned Davidson with mixed-precision applies until the
+    mixed floor, FP64 applies after (``davidson(dtype="mixed")``)."""
+    return dv.davidson(hamil, n_alpha, n_beta, tol=tol, max_iter=max_iter,
+                       dtype="mixed", diag=diag)

@@ -154,3 +154,19 @@ def test_davidson_thick_restart_matches_dense_eigh(fqe):
     print(f"thick restart: {res.iterations} iterations", flush=True)
     assert res.converged
     assert abs(res.eigenvalues[0] - w[0]) < 1e-10
+
+
+def test_davidson_mixed_then_fp64(fqe):
+    """Mixed-precision applies to the mixed floor, then FP64: the same
+    eigenvalue as dense diagonalisation to 1e-10 and a 1e-9 residual."""
+    from fqe_amd.algorithm import davidson as dv
+    rng = numpy.random.default_rng(29)
+    norb, na, nb = 7, 3, 3
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    w = numpy.linalg.eigvalsh(fo.dense_matrix(h1, h2, g))
+    ham = fqe.get_restricted_hamiltonian((h1, h2), e_0=-0.5)
+    res = dv.davidson(ham, na, nb, tol=1e-9, dtype="mixed")
+    print(f"mixed-then-FP64: {res.iterations} iterations", flush=True)
+    assert res.converged
+    assert abs(res.eigenvalues[0] - (w[0] - 0.5)) < 1e-10
