@@ -385,6 +385,17 @@ int fqehip_batch_launch(fqehip_batch* batch, void* stream);
 int fqehip_batch_nodes(const fqehip_batch* batch);
 void fqehip_batch_destroy(fqehip_batch* batch);
 
+/* ---------------------------------------------------------------------
+ * Complex Gram matrix of two blocks of k complex128 vectors of length n:
+ * d_out[p][q] = sum_i conj(a[p][i]) b[q][i] (complex128 [k][k]), FP64
+ * matrix cores, fixed summation order.  The 2-RDM's dvec overlap
+ * (src/fqe/fqe_data.py:1767-1782).  d_workspace:
+ * fqehip_gram_workspace_bytes(n, k) bytes; k <= 4096.
+ * ------------------------------------------------------------------- */
+int64_t fqehip_gram_workspace_bytes(int64_t n, int k);
+int fqehip_gram(int64_t n, int k, const double* d_a, const double* d_b,
+                double* d_out, double* d_workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,9 @@ _SIGNATURES = {
                                 c_void_p, c_void_p, c_void_p, c_void_p,
                                 c_void_p]),
     "fqehip_block_dot_workspace_bytes": (c_int64, [c_int]),
+    "fqehip_gram_workspace_bytes": (c_int64, [c_int64, c_int]),
+    "fqehip_gram": (c_int, [c_int64, c_int, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p]),
     "fqehip_block_dot": (c_int, [c_int64, c_int, c_void_p, c_int64, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     "fqehip_block_combine": (c_int, [c_int64, c_int, c_void_p, c_int64,

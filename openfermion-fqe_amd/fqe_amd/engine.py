@@ -257,3 +257,18 @@ def block_dots(rows: torch.Tensor, x: torch.Tensor) -> numpy.ndarray:
               _lib.ptr(out), _lib.ptr(ws), _lib.stream_ptr())
     return out.cpu().numpy()
 
+
+def gram(a: torch.Tensor, b: torch.Tensor) -> numpy.ndarray:
+    """G[p, q] = sum_i conj(a[p, i]) b[q, i] for complex128 device blocks
+    a, b of shape [k, n] (fqehip_gram, FP64 matrix cores)."""
+    k, n = a.shape
+    assert b.shape == (k, n)
+    assert a.dtype == torch.complex128 and b.dtype == torch.complex128
+    assert a.is_contiguous() and b.is_contiguous()
+    ws = torch.empty(max(int(_lib.call("fqehip_gram_workspace_bytes", n, k)),
+                         16), dtype=torch.uint8, device=a.device)
+    out = torch.empty((k, k), dtype=torch.complex128, device=a.device)
+    _lib.call("fqehip_gram", n, k, _lib.ptr(a), _lib.ptr(b), _lib.ptr(out),
+              _lib.ptr(ws), _lib.stream_ptr())
+    return out.cpu().numpy()
+

@@ -236,17 +236,15 @@ class FqeData:
         """1- and 2-RDMs in the reference's convention
         (``src/fqe/fqe_data.py:1767-1782``):
         rdm2[j, k, i, l] = -<E_ij bra|E_kl ket> + delta_ki rdm1[j, l].
-        The norb^2 x norb^2 overlap matrix of the two dvecs is formed with
-        the FP64 block inner-product kernel, one column per ket pair."""
+        The norb^2 x norb^2 overlap matrix of the two dvecs is one FP64
+        matrix-core Gram product (fqehip_gram)."""
         bra = self if bradata is None else bradata
         norb = self.norb()
         n2 = norb * norb
         d = self._dvec128()
         d2 = d if bradata is None else bra._dvec128()
         out1 = engine.block_dots(d2, self._coeff128()).reshape(norb, norb).T
-        m = numpy.empty((n2, n2), dtype=numpy.complex128)
-        for q in range(n2):
-            m[:, q] = engine.block_dots(d2, d[q].contiguous())
+        m = engine.gram(d2, d)
         out2 = -m.reshape((norb,) * 4).transpose(1, 2, 0, 3).copy()
         for i in range(norb):
             out2[:, i, i, :] += out1

@@ -170,3 +170,18 @@ def test_davidson_mixed_then_fp64(fqe):
     print(f"mixed-then-FP64: {res.iterations} iterations", flush=True)
     assert res.converged
     assert abs(res.eigenvalues[0] - (w[0] - 0.5)) < 1e-10
+
+
+@pytest.mark.parametrize("k,n", [(37, 10001), (16, 4096), (5, 3), (144, 9000)])
+def test_gram_matches_numpy(fqe, k, n):
+    """fqehip_gram (the 2-RDM's dvec overlap) against numpy, with ragged
+    k (not a multiple of 16) and n (not a multiple of 4 or of a chunk)."""
+    import torch
+    from fqe_amd import _lib, engine
+    rng = numpy.random.default_rng(k + n)
+    a = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
+    b = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
+    dev = _lib.device()
+    g = engine.gram(torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev))
+    ref = a.conj() @ b.T
+    assert numpy.max(numpy.abs(g - ref)) < 1e-11 * max(1.0, numpy.abs(ref).max())
