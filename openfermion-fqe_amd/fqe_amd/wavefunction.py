@@ -13,6 +13,8 @@ the Taylor series does not converge).
 from __future__ import annotations
 
 import copy
+import json
+import os
 import math
 from typing import Dict, KeysView, List, Optional, Tuple, Union
 
@@ -242,6 +244,64 @@ class Wavefunction:
         if numpy.abs(hamil.e_0() * time) > 1.e-15:
             out.scale(numpy.exp(-1.j * time * hamil.e_0()))
         return out
+
+    # -- on-disk format -------------------------------------------------------
+    # The reference pickles [symmetry map, conserved, conserve_spin,
+    # conserve_number, norb, [key, FqeData]...] (src/fqe/wavefunction.py:
+    # 726-765).  Here the same fields go into a versioned numpy .npz with
+    # allow_pickle=False: a JSON header ("fqe_amd.wavefunction", version 1)
+    # plus one complex array per sector, so reading a file executes nothing
+    # from it.
+    _FORMAT = "fqe_amd.wavefunction"
+    _VERSION = 1
+
+    def save(self, filename: str, path: Optional[str] = None) -> None:
+        """Write the wavefunction to path/filename (reference
+        ``src/fqe/wavefunction.py:748-765``)."""
+        keys = list(self._civec.keys())
+        header = {
+            "format": self._FORMAT, "version": self._VERSION,
+            "norb": self._norb, "conserve_spin": self._conserve_spin,
+            "conserve_number": self._conserve_number,
+            "conserved": self._conserved,
+            "symmetry_map": [[list(k), list(v)]
+                             for k, v in self._symmetry_map.items()],
+            "dtype": str(self._dtype),
+            "sectors": [[int(k[0]), int(k[1])] for k in keys],
+        }
+        arrays = {f"sector_{i}": numpy.ascontiguousarray(
+            self._civec[k].coeff) for i, k in enumerate(keys)}
+        full = os.path.join(path if path is not None else os.getcwd(),
+                            filename)
+        with open(full, "wb") as f:
+            numpy.savez(f, header=numpy.frombuffer(
+                json.dumps(header).encode(), dtype=numpy.uint8), **arrays)
+
+    def read(self, filename: str, path: Optional[str] = None) -> None:
+        """Initialise from a file written by ``save`` (reference
+        ``src/fqe/wavefunction.py:726-746``)."""
+        full = os.path.join(path if path is not None else os.getcwd(),
+                            filename)
+        with numpy.load(full, allow_pickle=False) as z:
+            header = json.loads(bytes(z["header"]).decode())
+            if header.get("format") != self._FORMAT:
+                raise ValueError(f"{full} is not a {self._FORMAT} file")
+            if header.get("version") != self._VERSION:
+                raise ValueError(f"unsupported {self._FORMAT} version "
+                                 f"{header.get('version')}")
+            self._norb = int(header["norb"])
+            self._conserve_spin = bool(header["conserve_spin"])
+            self._conserve_number = bool(header["conserve_number"])
+            self._conserved = dict(header["conserved"])
+            self._symmetry_map = {tuple(k): tuple(v)
+                                  for k, v in header["symmetry_map"]}
+            self._dtype = numpy.dtype(header["dtype"])
+            self._civec = {}
+            for i, (nele, m_s) in enumerate(header["sectors"]):
+                nalpha, nbeta = alpha_beta_electrons(nele, m_s)
+                sec = FqeData(nalpha, nbeta, self._norb, dtype=self._dtype)
+                sec.coeff = z[f"sector_{i}"]
+                self._civec[(nele, m_s)] = sec
 
     def print_wfn(self, threshold: float = 0.001, fmt: str = 'str') -> None:
         for key, sec in self._civec.items():

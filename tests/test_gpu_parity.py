@@ -586,3 +586,24 @@ def test_rdm1_contracts_to_the_one_body_energy(fqe):
     _, r2 = ket.rdm12()
     e2 = numpy.vdot(c, fo.apply_12(c, numpy.zeros((norb, norb)), h2, g))
     assert abs(numpy.einsum("ijkl,ijkl->", h2, r2) - e2) < 1e-11
+
+
+@pytest.mark.parametrize("dtype", [numpy.complex128, numpy.complex64])
+def test_wavefunction_save_read_round_trip(fqe, tmp_path, dtype):
+    """Wavefunction.save / read (reference src/fqe/wavefunction.py:726-765)
+    in the pickle-free format: same sectors, norb, conserved quantities and
+    coefficients; a foreign file is refused."""
+    wfn = fqe.Wavefunction([[4, 0, 5], [4, 2, 5], [4, -2, 5]], dtype=dtype)
+    wfn.set_wfn(strategy="random")
+    wfn.save("w.npz", path=str(tmp_path))
+    other = fqe.Wavefunction([[2, 0, 3]])
+    other.read("w.npz", path=str(tmp_path))
+    assert other.norb() == 5 and set(other.sectors()) == set(wfn.sectors())
+    assert other._conserved == wfn._conserved
+    for key in wfn.sectors():
+        got = other.get_coeff(key)
+        assert got.dtype == numpy.dtype(dtype)
+        assert numpy.array_equal(got, wfn.get_coeff(key))
+    numpy.savez(str(tmp_path / "bad.npz"), x=numpy.zeros(3))
+    with pytest.raises((ValueError, KeyError)):
+        other.read("bad.npz", path=str(tmp_path))
