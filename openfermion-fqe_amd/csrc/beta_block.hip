@@ -106,6 +106,8 @@ k_beta_split(int64_t lenb, int64_t ldh, const double* __restrict__ hcol,
 constexpr int kGT = 128;                  // tile rows and columns
 constexpr int kGK = 32;                   // K step
 constexpr int kGStride = kGK + 8;         // bf16 per LDS row (conflict-free)
+constexpr int kXcds = 8;                  // XCDs of an MI355X
+constexpr int kGroupRows = 8;             // tile rows per group of the order
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p,
                                                        int64_t bytes) {
@@ -118,14 +120,28 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
                    const __bf16* __restrict__ hhi,
                    const __bf16* __restrict__ hlo,
                    const float* __restrict__ coeff, float* __restrict__ out,
-                   const int32_t* __restrict__ done) {
+                   const int32_t* __restrict__ done, int ta, int tb,
+                   int per_xcd) {
   __shared__ __align__(16) __bf16 sa[4][kGT * kGStride];  // hr, lr, hi, li
   if (done && *done) return;
+  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin,
+  // so workgroup L runs on XCD L % 8.  Each XCD takes one contiguous run of
+  // tiles in a grouped order (kGroupRows tile rows at a time), so the C
+  // rows and Hbb rows it reads stay within its own L2.
+  const int xcd = static_cast<int>(blockIdx.x % kXcds);
+  const int t = xcd * per_xcd + static_cast<int>(blockIdx.x / kXcds);
+  if (t >= ta * tb) return;
+  const int grp = t / (kGroupRows * tb);
+  const int rows_here = (ta - grp * kGroupRows) < kGroupRows
+                            ? (ta - grp * kGroupRows) : kGroupRows;
+  const int within = t - grp * kGroupRows * tb;
+  const int tile_a = grp * kGroupRows + within % rows_here;
+  const int tile_b = within / rows_here;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
-  const int64_t a0 = static_cast<int64_t>(blockIdx.y) * kGT;
-  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kGT;
+  const int64_t a0 = static_cast<int64_t>(tile_a) * kGT;
+  const int64_t b0 = static_cast<int64_t>(tile_b) * kGT;
   const int wa = (wave >> 1) * 64;        // wave row offset in the tile
   const int wb = (wave & 1) * 64;         // wave column offset
   const __amdgpu_buffer_rsrc_t rc = rsrc(coeff, lena * lenb * 2 * sizeof(float));
@@ -256,11 +272,12 @@ int launch_beta_gemm(int64_t lena, int64_t lenb, int64_t ldh, const void* hhi,
                      const void* hlo, const float* coeff, float* out,
                      hipStream_t st, const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
-  dim3 grid(static_cast<unsigned>((lenb + kGT - 1) / kGT),
-            static_cast<unsigned>((lena + kGT - 1) / kGT));
-  k_beta_gemm_bf16x3<<<grid, 256, 0, st>>>(
+  const int ta = static_cast<int>((lena + kGT - 1) / kGT);
+  const int tb = static_cast<int>((lenb + kGT - 1) / kGT);
+  const int per_xcd = (ta * tb + kXcds - 1) / kXcds;
+  k_beta_gemm_bf16x3<<<static_cast<unsigned>(per_xcd * kXcds), 256, 0, st>>>(
       lena, lenb, ldh, static_cast<const __bf16*>(hhi),
-      static_cast<const __bf16*>(hlo), coeff, out, done);
+      static_cast<const __bf16*>(hlo), coeff, out, done, ta, tb, per_xcd);
   FQEHIP_CHECK_LAUNCH("k_beta_gemm_bf16x3");
   return FQEHIP_OK;
 }
