@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first copy this many bytes device-to-device (the "
                          "calibration kernel of HBM traffic counters)")
+    ap.add_argument("--split-tile", type=int, default=None,
+                    help="kernel of the split row pass (0, 16 or 32; "
+                         "default: the engine setting)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the unfused gather / GEMM / scatter kernels")
     ap.add_argument("--batched", action="store_true",
@@ -327,19 +330,23 @@ def kernel_work(op, graph, n, esz, mixed):
     """Algorithmic bytes and flops of one launch of each kernel.  GEMM
     flops are those of the real (npair+1) x npair pair matrix on 2n real
     columns; the mixed engine executes three BF16 products per flop.
-    Split apply: the alpha row kernel multiplies (npair+1) x (npair+1)
-    (extra h1' column), the beta-beta GEMM is lena x lenb x lenb complex
-    times real."""
+    Split apply: the alpha row kernel multiplies the rows of the pair
+    matrix that are scattered, nalpha * (norb - nalpha) excitations to
+    other strings plus the folded diagonal row (all npair + 1 rows with the
+    all-rows kernel), by npair + 1 columns (extra h1' column); the
+    beta-beta GEMM is lena x lenb x lenb complex times real."""
+    from fqe_amd import settings
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
     flops = 2.0 * (p + 1) * p * 2 * n
+    rows = p + 1 if settings.split_tile else nslot + 1
     lenb = graph.lenb()
     ldh = ((lenb + 127) // 128) * 128
     return {
         "beta_gemm": {"bytes": 2 * 2 * n * esz + 2 * ldh * ldh * 2,
                       "flops": 2.0 * 2 * n * lenb},
         "alpha_rows": {"bytes": (nslot + 3) * 2 * n * esz,
-                       "flops": 2.0 * (p + 1) * (p + 1) * 2 * n},
+                       "flops": 2.0 * rows * (p + 1) * 2 * n},
         "dvec_gather": {"bytes": (p + 1) * 2 * n * esz},
         "pair_gemm": {"bytes": (2 * p + 1) * 2 * n * esz, "flops": flops},
         "scatter": {"bytes": (p + 1 + 1) * 2 * n * esz},
@@ -459,9 +466,11 @@ def main():
     import numpy
     import torch
     import fqe_amd as fqe
-    from fqe_amd import engine, _lib
+    from fqe_amd import engine, _lib, settings
     from fqe_amd import synthetic
 
+    if args.split_tile is not None:
+        settings.split_tile = args.split_tile
     if args.copy_probe:
         src = torch.empty(args.copy_probe, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)

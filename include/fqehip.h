@@ -174,14 +174,20 @@ typedef struct fqehip_sector {
   const uint16_t* hbb_hi;
   const uint16_t* hbb_lo;
   int64_t ldh;
-  /* Tile shape of the complex64 row kernel when it also scatters beta
-   * excitations (hbb_hi NULL): FQEHIP_TILE_ROWS4 (default, 0 selects it)
-   * or FQEHIP_TILE_ROWS8 (4 or 8 waves per alpha row). */
+  /* Tile shape of the complex64 row kernel: with beta scatter (hbb_hi
+   * NULL) FQEHIP_TILE_ROWS4 (default, 0 selects it) or FQEHIP_TILE_ROWS8
+   * (4 or 8 waves per alpha row); on the split path 0 selects the kernel
+   * over the live pair rows of each alpha string (excitations to another
+   * string, plus the folded diagonal row), FQEHIP_TILE_SPLIT_ALLROWS the
+   * kernel over all npair + 1 rows. */
   int tile;
 } fqehip_sector;
 
 #define FQEHIP_TILE_ROWS4 4
 #define FQEHIP_TILE_ROWS8 8
+/* split path (hbb set): MFMAs over all npair + 1 pair rows instead of the
+ * live rows of each alpha string (the default, tile 0) */
+#define FQEHIP_TILE_SPLIT_ALLROWS 16
 
 typedef struct fqehip_operator {
   int complex_op;   /* 0: A is (npair+1) x npair; 1: 2(npair+1) x 2npair */

@@ -582,7 +582,8 @@ int check_sector_op(int precision, const fqehip_sector* s,
   FQEHIP_CHECK_ARG(op->complex_op == 0 || op->complex_op == 1,
                    "complex_op=%d", op->complex_op);
   FQEHIP_CHECK_ARG(s->tile == 0 || s->tile == FQEHIP_TILE_ROWS4 ||
-                       s->tile == FQEHIP_TILE_ROWS8, "tile=%d", s->tile);
+                       s->tile == FQEHIP_TILE_ROWS8 ||
+                       s->tile == FQEHIP_TILE_SPLIT_ALLROWS, "tile=%d", s->tile);
   return FQEHIP_OK;
 }
 

@@ -456,6 +456,226 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
   }
 }
 
+// ---------------- split kernel over the live pair rows ------------------------
+// Of the npair + 1 rows of Y = A' D only those whose alpha excitation of
+// the row's string ia is a different string (sbase >= 0, nalpha * (norb -
+// nalpha) of them) and the folded row npair are scattered.  This kernel
+// compacts them per alpha row into at most kLiveRows rows: the A'
+// fragments of the live rows are rebuilt in registers per alpha row, so
+// the chunk loop runs half the MFMAs of the all-rows kernel, and each
+// lane's scatter targets (slot base and sign of its 4 output rows) are
+// computed once per alpha row and kept in registers.
+constexpr int kLiveRows = 64;               // 4 tiles of 16 rows
+
+template <int NW, int WPE>
+__global__ void __launch_bounds__(NW * 64, WPE)
+k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
+             const double* __restrict__ amat,
+             const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+             const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
+             int nslot, const float* __restrict__ coeff,
+             float* __restrict__ slots, float* __restrict__ rowacc,
+             const int32_t* __restrict__ done) {
+  constexpr int kT = NW * 64;
+  constexpr int kRT = kLiveRows / 16 / NW;     // live row tiles per wave
+  constexpr int kGroups = kT / kBfTbC;         // gather pair groups of 8
+  constexpr int kGIter = kBfKPad / (8 * kGroups);
+  static_assert(kRT >= 1 && kRT * NW * 16 == kLiveRows, "row tiles");
+  static_assert(kGIter >= 1 && kGIter * 8 * kGroups == kBfKPad, "gather");
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int Kg = K + 1;                        // gathered rows: pairs + C
+  __bf16* dhi = reinterpret_cast<__bf16*>(smem);
+  __bf16* dlo = dhi + kBfCols * kBfStride;
+  double* drow = reinterpret_cast<double*>(dlo + kBfCols * kBfStride);
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(drow + kBfKPad);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  int32_t* lpair = rt.ssgn + kRowTabs;         // [kLiveRows] pair of live row
+
+  const int t = tid % kBfTbC;                  // gather column
+  const int g = tid / kBfTbC;                  // gather pair group
+  const int nchunks = static_cast<int>((lenb + kBfTbC - 1) / kBfTbC);
+  const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(float));
+  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+    load_row_tables<float>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                           kT, K);
+    __syncthreads();
+    // live rows: the pairs with sbase >= 0 in increasing order (wave 0,
+    // ballot compaction), then row npair, then padding (-1)
+    if (wave == 0) {
+      int base = 0;
+      for (int p0 = 0; p0 < K; p0 += 64) {
+        const int p = p0 + lane;
+        const bool live = p < K && rt.sbase[p] >= 0;
+        const uint64_t m = __ballot(live);
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (live && pos < kLiveRows) lpair[pos] = p;
+        base += __popcll(m);
+      }
+      if (lane >= base) lpair[lane] = lane == base ? npair : -1;
+    }
+    // row npair of A' for this alpha string: sum over occupied orbitals i
+    // (E^a_ii ia = ia, marked -2 in the row tables) of A'[ii][:]
+    for (int k = tid; k < Kg; k += kT) {
+      double v = 0.0;
+      for (int i = 0; i < norb; ++i) {
+        const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
+        if (rt.sbase[dp] == -2)
+          v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
+                     : amat[static_cast<int64_t>(npair) * K + dp];
+      }
+      drow[k] = v;
+    }
+    __syncthreads();
+    // A' fragments of the live rows (bf16 hi / lo split), the scatter
+    // targets of this lane's output rows and the alpha gather entries of
+    // this thread's pair rows
+    bf16x8 ahi[kRT][kBfKSteps], alo[kRT][kBfKSteps];
+    int64_t obase[kRT][4];                     // -1 none, -3 rowacc
+    float osgn[kRT][4];
+#pragma unroll
+    for (int r = 0; r < kRT; ++r) {
+      const int p = lpair[(wave * kRT + r) * 16 + (lane & 15)];
+#pragma unroll
+      for (int kt = 0; kt < kBfKSteps; ++kt) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int kk = kt * 32 + 8 * (lane >> 4) + j;
+          double v = 0.0;
+          if (p == npair) {
+            if (kk < Kg) v = drow[kk];
+          } else if (p >= 0) {
+            if (kk < K) v = amat[static_cast<int64_t>(p) * K + kk];
+            else if (kk == K) v = amat[static_cast<int64_t>(npair) * K + p];
+          }
+          const __bf16 h = to_bf16(static_cast<float>(v));
+          ahi[r][kt][j] = h;
+          alo[r][kt][j] = to_bf16(static_cast<float>(v - static_cast<double>(from_bf16(h))));
+        }
+      }
+      // f32 MFMA layout: item i of lane l is row 4*(l/16)+i, col l%16
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int po = lpair[(wave * kRT + r) * 16 + 4 * (lane >> 4) + i];
+        obase[r][i] = po == npair ? -3 : (po < 0 ? -1 : rt.sbase[po]);
+        osgn[r][i] = po >= 0 && po < npair ? static_cast<float>(rt.ssgn[po]) : 1.0f;
+      }
+    }
+    int32_t aof[kGIter][8];
+    float asg[kGIter][8];
+#pragma unroll
+    for (int it = 0; it < kGIter; ++it)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int q = 8 * (g + it * kGroups) + j;
+        const int32_t off = q < Kg ? rt.aoff[q] : -1;
+        aof[it][j] = off;
+        asg[it][j] = off >= 0 ? static_cast<float>(rt.asgn[q]) : 0.0f;
+      }
+    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(float));
+    float* racc = rowacc + ia * lenb * 2;
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
+      const int64_t ib0 = static_cast<int64_t>(chunk) * kBfTbC;
+      {
+        const int ib = static_cast<int>(ib0) + t;
+        const bool colok = ib < lenb;
+#pragma unroll
+        for (int it = 0; it < kGIter; ++it) {
+          const int q0 = 8 * (g + it * kGroups);
+          bf16x8 hr, lr, hi, li;
+          float2 ca[8], cb[8];
+          int32_t bt[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int q = q0 + j;
+            bt[j] = bload_i32(rpb, (colok && q < K)
+                                       ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int32_t off = aof[it][j];
+            ca[j] = bload_f2(rc, (colok && off >= 0) ? (off + 2 * ib) * 4 : -1);
+            cb[j] = bload_f2(rc, bt[j] ? crow_b + 8 * (iabs(bt[j]) - 1) : -1);
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const float sb = 2.0f * sgnf(bt[j]);
+            const float re = asg[it][j] * ca[j].x + sb * cb[j].x;
+            const float im = asg[it][j] * ca[j].y + sb * cb[j].y;
+            const __bf16 h0 = to_bf16(re), h1 = to_bf16(im);
+            hr[j] = h0;
+            lr[j] = to_bf16(re - from_bf16(h0));
+            hi[j] = h1;
+            li[j] = to_bf16(im - from_bf16(h1));
+          }
+          *reinterpret_cast<bf16x8*>(dhi + t * kBfStride + q0) = hr;
+          *reinterpret_cast<bf16x8*>(dlo + t * kBfStride + q0) = lr;
+          *reinterpret_cast<bf16x8*>(dhi + (kBfTbC + t) * kBfStride + q0) = hi;
+          *reinterpret_cast<bf16x8*>(dlo + (kBfTbC + t) * kBfStride + q0) = li;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        floatx4 are[kRT], aim[kRT];
+#pragma unroll
+        for (int r = 0; r < kRT; ++r) {
+          are[r] = floatx4{0, 0, 0, 0};
+          aim[r] = floatx4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int kt = 0; kt < kBfKSteps; ++kt) {
+          const int kq = kt * 32 + 8 * (lane >> 4);
+          const int cre = h * 16 + (lane & 15);
+          const int cim = kBfTbC + cre;
+          const bf16x8 bhr = *reinterpret_cast<const bf16x8*>(dhi + cre * kBfStride + kq);
+          const bf16x8 blr = *reinterpret_cast<const bf16x8*>(dlo + cre * kBfStride + kq);
+          const bf16x8 bhi = *reinterpret_cast<const bf16x8*>(dhi + cim * kBfStride + kq);
+          const bf16x8 bli = *reinterpret_cast<const bf16x8*>(dlo + cim * kBfStride + kq);
+#pragma unroll
+          for (int r = 0; r < kRT; ++r) {
+            are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhr, are[r], 0, 0, 0);
+            aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhi, aim[r], 0, 0, 0);
+            are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], blr, are[r], 0, 0, 0);
+            aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bli, aim[r], 0, 0, 0);
+            are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhr, are[r], 0, 0, 0);
+            aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhi, aim[r], 0, 0, 0);
+          }
+        }
+        const int64_t jb = ib0 + h * 16 + (lane & 15);
+        if (jb < lenb) {
+#pragma unroll
+          for (int r = 0; r < kRT; ++r)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int64_t sb = obase[r][i];
+              if (sb >= 0) {
+                *reinterpret_cast<float2*>(slots + sb + 2 * jb) =
+                    make_float2(osgn[r][i] * are[r][i], osgn[r][i] * aim[r][i]);
+              } else if (sb == -3) {
+                float2* d = reinterpret_cast<float2*>(racc + 2 * jb);
+                float2 o = *d;
+                o.x += are[r][i];
+                o.y += aim[r][i];
+                *d = o;
+              }
+            }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------- FP64: v_mfma_f64_16x16x4f64 -----------------------------
 // One workgroup per CU (the FP64 row accumulator fills the LDS); the D tile
 // is double-buffered so the gather of chunk c+1 (coefficients loaded one
@@ -673,6 +893,11 @@ size_t bf16_lds_bytes(int64_t lenb, bool split = false) {
          (split ? kBfKPad * sizeof(double) : 2 * lenb * sizeof(float));
 }
 
+size_t split_live_lds_bytes() {
+  return 2 * kBfCols * kBfStride * sizeof(__bf16) + kBfKPad * sizeof(double) +
+         row_tables_bytes() + kLiveRows * sizeof(int32_t);
+}
+
 size_t f64_lds_bytes(int64_t lenb) {
   return 2 * kF64Tile * sizeof(double) + row_tables_bytes() +
          2 * lenb * sizeof(double);
@@ -717,8 +942,23 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
       set_error("the split apply needs complex64 and npair + 1 <= %d", kBfKPad);
       return FQEHIP_ERR_INVALID;
     }
-    // three workgroups per CU: the split kernel needs no LDS accumulator
-    // row, and the higher occupancy outweighs a few spilled registers
+    if (tile != FQEHIP_TILE_SPLIT_ALLROWS) {
+      // live rows: at most norb^2/4 excitations to other strings plus row
+      // npair (57 for the largest sector here, norb 15)
+      if (norb * norb / 4 + 1 > kLiveRows) {
+        set_error("the live-rows kernel needs norb^2/4 + 1 <= %d", kLiveRows);
+        return FQEHIP_ERR_INVALID;
+      }
+      k_split_live<4, 3><<<fused_grid(lena), 256, split_live_lds_bytes(), st>>>(
+          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+          static_cast<const float*>(coeff), static_cast<float*>(slots),
+          static_cast<float*>(rowacc), done);
+      FQEHIP_CHECK_LAUNCH("k_split_live");
+      return FQEHIP_OK;
+    }
+    // all npair + 1 rows (the live-rows kernel's cross-check); three
+    // workgroups per CU: no LDS accumulator row, and the higher occupancy
+    // outweighs a few spilled registers
     k_fused_bf16x3<4, 3, true><<<fused_grid(lena), 256, bf16_lds_bytes(lenb, true), st>>>(
         norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, fwb, sla, nslot,
         static_cast<const float*>(coeff), static_cast<float*>(slots),

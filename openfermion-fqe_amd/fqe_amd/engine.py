@@ -171,6 +171,7 @@ def sector_struct(graph, op: PreparedOperator,
                     hbb[2] if hbb else 0)
     if hbb:
         keep += [hbb[0], hbb[1]]
+        s.tile = int(settings.split_tile)
     return s, tuple(keep)
 
 

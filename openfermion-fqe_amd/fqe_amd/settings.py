@@ -22,3 +22,8 @@ fused = True
 # one BF16x3 GEMM and an alpha-only row kernel (no beta scatter).  Off, or
 # for other operators, the row kernel also scatters beta excitations.
 split = True
+
+# Kernel of the split row pass: 0 = MFMAs over the live pair rows of each
+# alpha string only (excitations to another string plus the folded
+# diagonal row); 16 = MFMAs over all npair + 1 pair rows.
+split_tile = 0

@@ -501,6 +501,38 @@ def test_split_apply(fqe, na, nb, norb):
     assert rel(split, rows) < MIXED_TOL
 
 
+@pytest.mark.parametrize("tile", [16])
+@pytest.mark.parametrize("na,nb,norb", [(3, 0, 7), (5, 5, 10), (1, 1, 3),
+                                        (2, 6, 8), (6, 5, 12), (7, 7, 14),
+                                        (0, 4, 9), (8, 7, 15)])
+def test_split_apply_live_rows(fqe, na, nb, norb, tile):
+    """The split row kernel over the live pair rows (the default) computes
+    the same products as the all-rows kernel (tile 16; same operands per
+    output element), so their applies are identical, bit for bit, and
+    agree with the oracle (sectors small enough for the oracle; the full-size sectors,
+    norb 14 and 15 at half filling, are checked against the all-rows
+    kernel only)."""
+    from fqe_amd import settings
+    print(f"split live rows {tile}: {na} {nb} {norb}", flush=True)
+    rng = numpy.random.default_rng(norb * 17 + na * 3 + nb)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    ref = fo.apply_12(c, h1, h2, g) if g.lena * g.lenb <= 200000 else None
+    d = fqe.FqeData(na, nb, norb, dtype=numpy.complex64)
+    d.coeff = c
+    out = d.apply((h1, h2)).coeff
+    saved = settings.split_tile
+    settings.split_tile = tile
+    try:
+        base = d.apply((h1, h2)).coeff
+    finally:
+        settings.split_tile = saved
+    if ref is not None:
+        assert rel(out, ref) < MIXED_TOL
+    assert numpy.array_equal(out, base)
+
+
 # --------------------------------------------------------------------------
 # Chebyshev propagator (reference src/fqe/wavefunction.py:570-602; the
 # reference's tests tests/evolution_test.py:777-808 and
