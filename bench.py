@@ -53,8 +53,12 @@ def parse():
                     help="first copy this many bytes device-to-device (the "
                          "calibration kernel of HBM traffic counters)")
     ap.add_argument("--split-tile", type=int, default=None,
-                    help="kernel of the split row pass (0, 16 or 32; "
-                         "default: the engine setting)")
+                    help="kernel of the complex64 split row pass (0: live "
+                         "pair rows, 16: all pair rows; default: the engine "
+                         "setting)")
+    ap.add_argument("--no-split-fp64", action="store_true",
+                    help="FP64 engine without the split apply (the FP64 row "
+                         "kernel also scatters beta excitations)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the unfused gather / GEMM / scatter kernels")
     ap.add_argument("--batched", action="store_true",
@@ -245,7 +249,7 @@ class EngineRun:
         self.n = self.c.numel()
         esz = 8 if self.prec == _lib.C128 else 4
         self.y_ptr = self.ws.data_ptr() + op.npair * 2 * self.n * esz
-        self.split = bool(self.s.hbb_hi)
+        self.split = bool(self.s.hbb_hi) or bool(self.s.hbb_f64)
         self.rowacc_ptr = self.ws.data_ptr() + self.s.nslot * 2 * self.n * esz
         self.names = (["fused_rows", "slot_reduce"] if self.fused else
                       ["dvec_gather", "pair_gemm", "scatter"])
@@ -308,9 +312,14 @@ class EngineRun:
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
             ev[0].record()
             for _ in range(steps):
-                L.call("fqehip_beta_gemm", s.lena, s.lenb, s.hbb_hi, s.hbb_lo,
-                       s.ldh, L.ptr(self.c), self.rowacc_ptr, None,
-                       L.stream_ptr())
+                if self.prec == L.C128:
+                    L.call("fqehip_beta_gemm_f64", s.lena, s.lenb, s.hbb_f64,
+                           L.ptr(self.c), self.rowacc_ptr, None,
+                           L.stream_ptr())
+                else:
+                    L.call("fqehip_beta_gemm", s.lena, s.lenb, s.hbb_hi,
+                           s.hbb_lo, s.ldh, L.ptr(self.c), self.rowacc_ptr,
+                           None, L.stream_ptr())
             ev[1].record()
             torch.cuda.synchronize()
             tb = ev[0].elapsed_time(ev[1]) / steps
@@ -334,16 +343,18 @@ def kernel_work(op, graph, n, esz, mixed):
     matrix that are scattered, nalpha * (norb - nalpha) excitations to
     other strings plus the folded diagonal row (all npair + 1 rows with the
     all-rows kernel), by npair + 1 columns (extra h1' column); the
-    beta-beta GEMM is lena x lenb x lenb complex times real."""
+    beta-beta GEMM is lena x lenb x lenb complex times real (bfloat16
+    hi/lo copies of Hbb for the mixed engine, FP64 columns for FP64)."""
     from fqe_amd import settings
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
     flops = 2.0 * (p + 1) * p * 2 * n
-    rows = p + 1 if settings.split_tile else nslot + 1
+    rows = p + 1 if (mixed and settings.split_tile) else nslot + 1
     lenb = graph.lenb()
     ldh = ((lenb + 127) // 128) * 128
     return {
-        "beta_gemm": {"bytes": 2 * 2 * n * esz + 2 * ldh * ldh * 2,
+        "beta_gemm": {"bytes": 2 * 2 * n * esz + (2 * ldh * ldh * 2 if mixed
+                                                  else lenb * lenb * 8),
                       "flops": 2.0 * 2 * n * lenb},
         "alpha_rows": {"bytes": (nslot + 3) * 2 * n * esz,
                        "flops": 2.0 * rows * (p + 1) * 2 * n},
@@ -471,6 +482,8 @@ def main():
 
     if args.split_tile is not None:
         settings.split_tile = args.split_tile
+    if args.no_split_fp64:
+        settings.split_fp64 = False
     if args.copy_probe:
         src = torch.empty(args.copy_probe, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)

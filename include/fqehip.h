@@ -181,6 +181,12 @@ typedef struct fqehip_sector {
    * string, plus the folded diagonal row), FQEHIP_TILE_SPLIT_ALLROWS the
    * kernel over all npair + 1 rows. */
   int tile;
+  /* FP64 beta-beta block of the same operator, column-major [lenb][lenb]
+   * (entry [Ib][Ib'] = Hbb[Ib'][Ib], the hcol output of fqehip_beta_block).
+   * When set, complex128 applies take the split path (FP64 beta-beta GEMM,
+   * FP64 row kernel over the live pair rows); NULL keeps the FP64 row
+   * kernel that also scatters beta excitations. */
+  const double* hbb_f64;
 } fqehip_sector;
 
 #define FQEHIP_TILE_ROWS4 4
@@ -237,6 +243,16 @@ int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b,
 int fqehip_beta_gemm(int64_t lena, int64_t lenb, const uint16_t* d_hhi,
                      const uint16_t* d_hlo, int64_t ldh, const void* d_coeff,
                      void* d_out, const int32_t* d_done, void* stream);
+
+/* out[Ia][Ib'] = sum_Ib C[Ia][Ib] Hbb[Ib'][Ib] for complex128 C on the FP64
+ * matrix cores, from the FP64 columns d_hcol ([Ib][Ib'], the hcol output
+ * of fqehip_beta_block): the beta-beta half of the complex128 split apply.
+ * Replaces the same-spin beta part of lm_apply_array12_same_spin_opt
+ * (src/fqe/lib/fqe_data.c:668-720) as one dense GEMM.  Skipped when
+ * d_done and *d_done. */
+int fqehip_beta_gemm_f64(int64_t lena, int64_t lenb, const double* d_hcol,
+                         const void* d_coeff, void* d_out,
+                         const int32_t* d_done, void* stream);
 int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
                             const fqehip_operator* op, const void* d_coeff,
                             void* d_workspace, const int32_t* d_done,

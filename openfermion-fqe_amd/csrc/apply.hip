@@ -388,6 +388,7 @@ int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
 // Defined in fused.hip.
 bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb);
 bool split_supported(int npair, int64_t lena, int64_t lenb);
+bool split_f64_supported(int npair, int64_t lena, int64_t lenb);
 int launch_fused(int precision, int norb, int npair, int64_t lena, int64_t lenb,
                  const double* a, const int32_t* pta, const int32_t* ptb,
                  const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
@@ -398,6 +399,10 @@ bool beta_gemm_shape_ok(int64_t lena, int64_t lenb, int64_t ldh);
 int launch_beta_gemm(int64_t lena, int64_t lenb, int64_t ldh, const void* hhi,
                      const void* hlo, const float* coeff, float* out,
                      hipStream_t st, const int32_t* done);
+bool beta_gemm_f64_shape_ok(int64_t lena, int64_t lenb);
+int launch_beta_gemm_f64(int64_t lena, int64_t lenb, const double* hcol,
+                         const double* coeff, double* out, hipStream_t st,
+                         const int32_t* done);
 int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
                        const void* slots, const void* rowacc, double ar,
                        double ai, double br, double bi, const void* cin,
@@ -595,9 +600,13 @@ size_t elem_size(int precision) {
 
 extern "C" int fqehip_fused_supported(int precision, const fqehip_sector* s,
                                       const fqehip_operator* op) {
-  return s && op && op->a && !op->h1 && !op->complex_op && s->fwd_a &&
-         s->fwd_b && s->slot_a &&
-         fused_supported(precision, s->npair, s->lena, s->lenb);
+  if (!(s && op && op->a && !op->h1 && !op->complex_op && s->fwd_a &&
+        s->fwd_b && s->slot_a))
+    return 0;
+  if (precision == FQEHIP_C128 && s->hbb_f64)
+    return split_f64_supported(s->npair, s->lena, s->lenb) &&
+           beta_gemm_f64_shape_ok(s->lena, s->lenb);
+  return fused_supported(precision, s->npair, s->lena, s->lenb);
 }
 
 extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
@@ -615,6 +624,17 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
   char* ws = static_cast<char*>(d_workspace);
   void* rowacc = ws + static_cast<int64_t>(s->nslot) * 2 * n * elem_size(precision);
   hipStream_t st = as_stream(stream);
+  if (precision == FQEHIP_C128 && s->hbb_f64 != nullptr) {
+    // rowacc <- C Hbb^T (FP64), then the live alpha rows add their part
+    int rc = launch_beta_gemm_f64(s->lena, s->lenb, s->hbb_f64,
+                                  static_cast<const double*>(d_coeff),
+                                  static_cast<double*>(rowacc), st, d_done);
+    if (rc) return rc;
+    return launch_fused(precision, s->norb, s->npair, s->lena, s->lenb, op->a,
+                        s->ptab_a, s->ptab_b, s->fwd_a, s->fwd_b, s->slot_a,
+                        s->nslot, d_coeff, ws, rowacc, true, s->tile, st,
+                        d_done);
+  }
   const bool split = precision == FQEHIP_C64 && s->hbb_hi != nullptr;
   if (split) {
     FQEHIP_CHECK_ARG(s->hbb_lo != nullptr, "hbb_hi set without hbb_lo");

@@ -16,7 +16,9 @@
 //   k_beta_split:  transpose and split into bfloat16 hi + lo, zero-padded
 //                  to ldh x ldh (ldh a multiple of 128);
 //   k_beta_gemm_bf16x3: out[Ia][Ib'] = sum_Ib C[Ia][Ib] Hbb[Ib'][Ib] on
-//                  v_mfma_f32_16x16x32_bf16 with the three-product split.
+//                  v_mfma_f32_16x16x32_bf16 with the three-product split;
+//   k_beta_gemm_f64: the same product for complex128 C on
+//                  v_mfma_f64_16x16x4f64, from the FP64 columns of Hbb.
 #include "fqehip_common.h"
 
 namespace fqehip {
@@ -26,6 +28,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kBlockThreads = 256;
 
@@ -115,6 +118,25 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p,
                                            static_cast<int>(bytes), 0x00020000);
 }
 
+// XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
+// workgroup L runs on XCD L % 8.  Each XCD takes one contiguous run of
+// tiles in a grouped order (kGroupRows tile rows at a time), so the C rows
+// and Hbb rows it reads stay within its own L2.  Returns false for the
+// padding workgroups past ta * tb.
+__device__ __forceinline__ bool xcd_tile(int ta, int tb, int per_xcd,
+                                         int* tile_a, int* tile_b) {
+  const int xcd = static_cast<int>(blockIdx.x % kXcds);
+  const int t = xcd * per_xcd + static_cast<int>(blockIdx.x / kXcds);
+  if (t >= ta * tb) return false;
+  const int grp = t / (kGroupRows * tb);
+  const int rows_here = (ta - grp * kGroupRows) < kGroupRows
+                            ? (ta - grp * kGroupRows) : kGroupRows;
+  const int within = t - grp * kGroupRows * tb;
+  *tile_a = grp * kGroupRows + within % rows_here;
+  *tile_b = within / rows_here;
+  return true;
+}
+
 __global__ void __launch_bounds__(256, 1)
 k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
                    const __bf16* __restrict__ hhi,
@@ -124,19 +146,8 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
                    int per_xcd) {
   __shared__ __align__(16) __bf16 sa[4][kGT * kGStride];  // hr, lr, hi, li
   if (done && *done) return;
-  // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin,
-  // so workgroup L runs on XCD L % 8.  Each XCD takes one contiguous run of
-  // tiles in a grouped order (kGroupRows tile rows at a time), so the C
-  // rows and Hbb rows it reads stay within its own L2.
-  const int xcd = static_cast<int>(blockIdx.x % kXcds);
-  const int t = xcd * per_xcd + static_cast<int>(blockIdx.x / kXcds);
-  if (t >= ta * tb) return;
-  const int grp = t / (kGroupRows * tb);
-  const int rows_here = (ta - grp * kGroupRows) < kGroupRows
-                            ? (ta - grp * kGroupRows) : kGroupRows;
-  const int within = t - grp * kGroupRows * tb;
-  const int tile_a = grp * kGroupRows + within % rows_here;
-  const int tile_b = within / rows_here;
+  int tile_a, tile_b;
+  if (!xcd_tile(ta, tb, per_xcd, &tile_a, &tile_b)) return;
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
@@ -260,7 +271,152 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
       }
 }
 
+// FP64 GEMM tile: 128 rows (alpha strings) x 128 columns (target beta
+// strings) per workgroup of 8 waves in a 4 x 2 arrangement, 32 x 64 per
+// wave (2 x 4 MFMA tiles, real and imaginary accumulators), K (source beta
+// strings) in steps of 16.  Both operands are staged through LDS: the C
+// tile as separate real / imaginary planes [k][row] (the A operand), the
+// Hbb columns as [k][col] (the B operand, hcol[Ib][Ib'] = Hbb[Ib'][Ib] is
+// already K-major).  The next K step's global loads are in flight during
+// the MFMAs of the current one.
+constexpr int kFK = 16;                   // K step
+constexpr int kFS = kGT + 16;             // doubles per LDS row (padding)
+
+__global__ void __launch_bounds__(512, 1)
+k_beta_gemm_f64(int64_t lena, int64_t lenb, const double* __restrict__ hcol,
+                const double* __restrict__ coeff, double* __restrict__ out,
+                const int32_t* __restrict__ done, int ta, int tb,
+                int per_xcd) {
+  __shared__ __align__(16) double cre[kFK * kFS];
+  __shared__ __align__(16) double cim[kFK * kFS];
+  __shared__ __align__(16) double hb[kFK * kFS];
+  if (done && *done) return;
+  int tile_a, tile_b;
+  if (!xcd_tile(ta, tb, per_xcd, &tile_a, &tile_b)) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int64_t a0 = static_cast<int64_t>(tile_a) * kGT;
+  const int64_t b0 = static_cast<int64_t>(tile_b) * kGT;
+  const int wa = (wave >> 1) * 32;        // wave row offset in the tile
+  const int wb = (wave & 1) * 64;         // wave column offset
+  const __amdgpu_buffer_rsrc_t rc = rsrc(coeff, lena * lenb * 2 * sizeof(double));
+  const __amdgpu_buffer_rsrc_t rh = rsrc(hcol, lenb * lenb * sizeof(double));
+
+  // C loader: thread -> tile row tid/4, complex k 4*(tid%4)..+3
+  const int cr = tid >> 2;
+  const int ck = 4 * (tid & 3);
+  const int64_t arow = a0 + cr;
+  const bool rowok = arow < lena;
+  // Hbb loader: thread -> k row tid/32, columns 4*(tid%32)..+3
+  const int hk = tid >> 5;
+  const int hn = 4 * (tid & 31);
+  u32x4 cv[4];
+  uint64_t hv[4];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t k = k0 + ck + j;
+      const int off = (rowok && k < lenb)
+                          ? static_cast<int>((arow * lenb + k) * 2 * sizeof(double)) : -1;
+      cv[j] = __builtin_amdgcn_raw_buffer_load_b128(rc, off, 0, 0);
+    }
+    const int64_t k = k0 + hk;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t n = b0 + hn + j;
+      const int off = (k < lenb && n < lenb)
+                          ? static_cast<int>((k * lenb + n) * sizeof(double)) : -1;
+      const u32x2 y = __builtin_amdgcn_raw_buffer_load_b64(rh, off, 0, 0);
+      hv[j] = static_cast<uint64_t>(y[0]) | (static_cast<uint64_t>(y[1]) << 32);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint64_t re = static_cast<uint64_t>(cv[j][0]) | (static_cast<uint64_t>(cv[j][1]) << 32);
+      const uint64_t im = static_cast<uint64_t>(cv[j][2]) | (static_cast<uint64_t>(cv[j][3]) << 32);
+      cre[(ck + j) * kFS + cr] = __longlong_as_double(static_cast<long long>(re));
+      cim[(ck + j) * kFS + cr] = __longlong_as_double(static_cast<long long>(im));
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      hb[hk * kFS + hn + j] = __longlong_as_double(static_cast<long long>(hv[j]));
+  };
+
+  double4_t accr[2][4], acci[2][4];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      accr[r][c] = double4_t{0, 0, 0, 0};
+      acci[r][c] = double4_t{0, 0, 0, 0};
+    }
+
+  const int64_t nk = (lenb + kFK - 1) / kFK;
+  load(0);
+  for (int64_t ks = 0; ks < nk; ++ks) {
+    __syncthreads();                      // previous step's LDS reads done
+    store();
+    __syncthreads();
+    if (ks + 1 < nk) load((ks + 1) * kFK);  // next tiles in flight
+    // f64 MFMA operands: A[r = l%16][k = l/16], B[k = l/16][c = l%16]
+#pragma unroll
+    for (int kq = 0; kq < kFK / 4; ++kq) {
+      const int krow = (4 * kq + (lane >> 4)) * kFS;
+      double ar[2], ai[2], bv[4];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        ar[r] = cre[krow + wa + 16 * r + (lane & 15)];
+        ai[r] = cim[krow + wa + 16 * r + (lane & 15)];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) bv[c] = hb[krow + wb + 16 * c + (lane & 15)];
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          accr[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[r], bv[c], accr[r][c], 0, 0, 0);
+          acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[r], bv[c], acci[r][c], 0, 0, 0);
+        }
+    }
+  }
+  // f64 MFMA layout: item i of lane l is row l/16 + 4 i, column l%16
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t a = a0 + wa + 16 * r + (lane >> 4) + 4 * i;
+        const int64_t b = b0 + wb + 16 * c + (lane & 15);
+        if (a < lena && b < lenb) {
+          double2* d = reinterpret_cast<double2*>(out + (a * lenb + b) * 2);
+          *d = make_double2(accr[r][c][i], acci[r][c][i]);
+        }
+      }
+}
+
 }  // namespace
+
+bool beta_gemm_f64_shape_ok(int64_t lena, int64_t lenb) {
+  const int64_t lim = int64_t(1) << 31;
+  return lena * lenb * 2 * static_cast<int64_t>(sizeof(double)) < lim &&
+         lenb * lenb * static_cast<int64_t>(sizeof(double)) < lim;
+}
+
+int launch_beta_gemm_f64(int64_t lena, int64_t lenb, const double* hcol,
+                         const double* coeff, double* out, hipStream_t st,
+                         const int32_t* done) {
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  const int ta = static_cast<int>((lena + kGT - 1) / kGT);
+  const int tb = static_cast<int>((lenb + kGT - 1) / kGT);
+  const int per_xcd = (ta * tb + kXcds - 1) / kXcds;
+  k_beta_gemm_f64<<<static_cast<unsigned>(per_xcd * kXcds), 512, 0, st>>>(
+      lena, lenb, hcol, coeff, out, done, ta, tb, per_xcd);
+  FQEHIP_CHECK_LAUNCH("k_beta_gemm_f64");
+  return FQEHIP_OK;
+}
 
 bool beta_gemm_shape_ok(int64_t lena, int64_t lenb, int64_t ldh) {
   const int64_t lim = int64_t(1) << 31;
@@ -333,4 +489,21 @@ extern "C" int fqehip_beta_gemm(int64_t lena, int64_t lenb,
   return launch_beta_gemm(lena, lenb, ldh, d_hhi, d_hlo,
                           static_cast<const float*>(d_coeff),
                           static_cast<float*>(d_out), as_stream(stream), d_done);
+}
+
+extern "C" int fqehip_beta_gemm_f64(int64_t lena, int64_t lenb,
+                                    const double* d_hcol, const void* d_coeff,
+                                    void* d_out, const int32_t* d_done,
+                                    void* stream) {
+  clear_error();
+  FQEHIP_CHECK_ARG(lena >= 0 && lenb >= 0, "negative size");
+  FQEHIP_CHECK_ARG(beta_gemm_f64_shape_ok(lena, lenb),
+                   "lena=%lld lenb=%lld too large for 32-bit offsets",
+                   static_cast<long long>(lena), static_cast<long long>(lenb));
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  FQEHIP_CHECK_ARG(d_hcol && d_coeff && d_out, "NULL pointer");
+  return launch_beta_gemm_f64(lena, lenb, d_hcol,
+                              static_cast<const double*>(d_coeff),
+                              static_cast<double*>(d_out), as_stream(stream),
+                              d_done);
 }

@@ -827,6 +827,198 @@ k_fused_f64(int npair, int64_t lena, int64_t lenb,
   }
 }
 
+// ---------------- FP64 split kernel over the live pair rows ------------------
+// The complex128 counterpart of k_split_live: the beta-beta block has
+// already been written to the row accumulators by k_beta_gemm_f64, and this
+// kernel multiplies the live rows of A' (at most kLiveRows: four waves of
+// 16 rows) by D' = D^a + 2 D^b plus the C column on v_mfma_f64_16x16x4f64.
+// The D tile is double-buffered as in k_fused_f64: the gather of chunk c+1
+// (coefficients one chunk ahead, beta table entries two chunks ahead)
+// overlaps the MFMAs of chunk c, with one barrier per chunk.
+constexpr int kSF64Threads = 256;
+constexpr int kSF64Groups = kSF64Threads / kF64TbC;
+constexpr int kSF64GIter = kF64KPad / kSF64Groups;
+
+__global__ void __launch_bounds__(kSF64Threads, 2)
+k_split_live_f64(int norb, int compressed, int npair, int64_t lena,
+                 int64_t lenb, const double* __restrict__ amat,
+                 const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+                 const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
+                 int nslot, const double* __restrict__ coeff,
+                 double* __restrict__ slots, double* __restrict__ rowacc,
+                 const int32_t* __restrict__ done) {
+  static_assert(kLiveRows == 4 * 16, "one 16-row tile per wave");
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int Kg = K + 1;                     // gathered rows: pairs + C
+  const int ksteps = (Kg + 3) / 4;
+  double* dt = reinterpret_cast<double*>(smem);     // 2 x [kF64KPad][kF64Stride]
+  double* drow = dt + 2 * kF64Tile;
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(drow + kF64KPad);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  int32_t* lpair = rt.ssgn + kRowTabs;      // [kLiveRows] pair of live row
+
+  const int t = tid % kF64TbC;              // gather column
+  const int g = tid / kF64TbC;              // gather pair offset
+  const int nchunks = static_cast<int>((lenb + kF64TbC - 1) / kF64TbC);
+  const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(double));
+  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+    load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                            kSF64Threads, K);
+    __syncthreads();
+    // live rows: the pairs with sbase >= 0 in increasing order (wave 0,
+    // ballot compaction), then row npair, then padding (-1)
+    if (wave == 0) {
+      int base = 0;
+      for (int p0 = 0; p0 < K; p0 += 64) {
+        const int p = p0 + lane;
+        const bool live = p < K && rt.sbase[p] >= 0;
+        const uint64_t m = __ballot(live);
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (live && pos < kLiveRows) lpair[pos] = p;
+        base += __popcll(m);
+      }
+      if (lane >= base) lpair[lane] = lane == base ? npair : -1;
+    }
+    // row npair of A' for this alpha string (occupied diagonal pairs)
+    for (int k = tid; k < Kg; k += kSF64Threads) {
+      double v = 0.0;
+      for (int i = 0; i < norb; ++i) {
+        const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
+        if (rt.sbase[dp] == -2)
+          v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
+                     : amat[static_cast<int64_t>(npair) * K + dp];
+      }
+      drow[k] = v;
+    }
+    __syncthreads();
+    // A' fragment of this wave's 16 live rows (A[r = l%16][k = l/16]), the
+    // slot targets of this lane's output rows (l/16 + 4 i) and the alpha
+    // gather entries of this thread's pair rows
+    double areg[kF64KSteps];
+    {
+      const int p = lpair[wave * 16 + (lane & 15)];
+#pragma unroll
+      for (int kt = 0; kt < kF64KSteps; ++kt) {
+        const int kk = kt * 4 + (lane >> 4);
+        double v = 0.0;
+        if (p == npair) {
+          if (kk < Kg) v = drow[kk];
+        } else if (p >= 0) {
+          if (kk < K) v = amat[static_cast<int64_t>(p) * K + kk];
+          else if (kk == K) v = amat[static_cast<int64_t>(npair) * K + p];
+        }
+        areg[kt] = v;
+      }
+    }
+    // slot bases are even element offsets: bit 0 carries the sign
+    int64_t obase[4];                       // -1 none, -3 rowacc
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int po = lpair[wave * 16 + (lane >> 4) + 4 * i];
+      obase[i] = po == npair ? -3
+                             : (po < 0 ? -1 : rt.sbase[po] | (rt.ssgn[po] < 0 ? 1 : 0));
+    }
+    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(double));
+    double* racc = rowacc + ia * lenb * 2;
+
+    int32_t bt[kSF64GIter];                 // beta table entries (ahead)
+    auto load_bt = [&](int64_t ib0) {
+      const int ib = static_cast<int>(ib0) + t;
+#pragma unroll
+      for (int it = 0; it < kSF64GIter; ++it) {
+        const int q = g + it * kSF64Groups;
+        bt[it] = bload_i32(rpb, (ib < lenb && q < K)
+                                    ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+      }
+    };
+    // gather of one chunk into an LDS tile: coefficients loaded and stored
+    // in one pass (the MFMAs of the previous chunk are already issued)
+    auto gather = [&](int64_t ib0, double* tile) {
+      const int ib = static_cast<int>(ib0) + t;
+      const bool colok = ib < lenb;
+      constexpr int kHalf = kSF64GIter / 2;   // two passes bound the VGPRs
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double2 ca[kHalf], cb[kHalf];
+        int32_t as[kHalf];
+#pragma unroll
+        for (int j = 0; j < kHalf; ++j) {
+          const int it = h * kHalf + j;
+          const int q = g + it * kSF64Groups;
+          const int32_t off = q < Kg ? rt.aoff[q] : -1;
+          as[j] = off >= 0 ? rt.asgn[q] : 0;
+          ca[j] = bload_d2(rc, (colok && off >= 0) ? (off + 2 * ib) * 8 : -1);
+          cb[j] = bload_d2(rc, bt[it] ? crow_b + 16 * (iabs(bt[it]) - 1) : -1);
+        }
+#pragma unroll
+        for (int j = 0; j < kHalf; ++j) {
+          const int it = h * kHalf + j;
+          const int q = g + it * kSF64Groups;
+          const double sa = static_cast<double>(as[j]);
+          const double sb = bt[it] > 0 ? 2.0 : (bt[it] < 0 ? -2.0 : 0.0);
+          tile[q * kF64Stride + t] = sa * ca[j].x + sb * cb[j].x;
+          tile[q * kF64Stride + kF64TbC + t] = sa * ca[j].y + sb * cb[j].y;
+        }
+      }
+    };
+    // prologue: tile 0 in buffer 0, table entries of chunk 1 in registers
+    load_bt(0);
+    gather(0, dt);
+    if (nchunks > 1) load_bt(kF64TbC);
+    __syncthreads();
+
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
+      const int64_t ib0 = static_cast<int64_t>(chunk) * kF64TbC;
+      const double* cur = dt + (chunk & 1) * kF64Tile;
+      double* nxt = dt + ((chunk + 1) & 1) * kF64Tile;
+      const bool more = chunk + 1 < nchunks;
+      double4_t acc_re = {0, 0, 0, 0}, acc_im = {0, 0, 0, 0};
+#pragma unroll
+      for (int kt = 0; kt < kF64KSteps; ++kt) {
+        if (kt < ksteps) {
+          const double* r = cur + (kt * 4 + (lane >> 4)) * kF64Stride + (lane & 15);
+          acc_re = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[kt], r[0], acc_re, 0, 0, 0);
+          acc_im = __builtin_amdgcn_mfma_f64_16x16x4f64(areg[kt], r[kF64TbC], acc_im, 0, 0, 0);
+        }
+      }
+      if (more) {
+        gather(ib0 + kF64TbC, nxt);
+        if (chunk + 2 < nchunks) load_bt(ib0 + 2 * kF64TbC);
+      }
+      // f64 MFMA layout: item i of lane l is row l/16 + 4 i, col l%16
+      const int64_t jb = ib0 + (lane & 15);
+      if (jb < lenb) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int64_t sb = obase[i];
+          if (sb >= 0) {
+            const double sg = (sb & 1) ? -1.0 : 1.0;
+            *reinterpret_cast<double2*>(slots + (sb & ~int64_t(1)) + 2 * jb) =
+                make_double2(sg * acc_re[i], sg * acc_im[i]);
+          } else if (sb == -3) {
+            double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
+            double2 o = *d;
+            o.x += acc_re[i];
+            o.y += acc_im[i];
+            *d = o;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // slot reduction + epilogue
 // ---------------------------------------------------------------------------
@@ -898,6 +1090,11 @@ size_t split_live_lds_bytes() {
          row_tables_bytes() + kLiveRows * sizeof(int32_t);
 }
 
+size_t split_live_f64_lds_bytes() {
+  return (2 * kF64Tile + kF64KPad) * sizeof(double) + row_tables_bytes() +
+         kLiveRows * sizeof(int32_t);
+}
+
 size_t f64_lds_bytes(int64_t lenb) {
   return 2 * kF64Tile * sizeof(double) + row_tables_bytes() +
          2 * lenb * sizeof(double);
@@ -924,6 +1121,15 @@ bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb) {
          lena * lenb * 2 * 4 < lim;
 }
 
+// FP64 split: the live-rows kernel's LDS does not grow with lenb (the row
+// accumulator is the global rowacc), and its gather depth is kF64KMax.
+bool split_f64_supported(int npair, int64_t lena, int64_t lenb) {
+  const int64_t lim = int64_t(1) << 31;
+  return npair + 1 <= kF64KMax && static_cast<int64_t>(npair) * lenb * 4 < lim &&
+         lena * lenb * 2 * 8 < lim &&
+         split_live_f64_lds_bytes() <= kLdsLimit;
+}
+
 bool split_supported(int npair, int64_t lena, int64_t lenb) {
   return npair + 1 <= kBfKPad && fused_supported(FQEHIP_C64, npair, lena, lenb) &&
          bf16_lds_bytes(lenb, true) <= kLdsLimit;
@@ -937,6 +1143,20 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
                  const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
+  if (split && precision == FQEHIP_C128) {
+    if (!split_f64_supported(npair, lena, lenb) ||
+        norb * norb / 4 + 1 > kLiveRows) {
+      set_error("the FP64 split apply needs npair + 1 <= %d and norb^2/4 + 1 <= %d",
+                kF64KMax, kLiveRows);
+      return FQEHIP_ERR_INVALID;
+    }
+    k_split_live_f64<<<fused_grid(lena), kSF64Threads, split_live_f64_lds_bytes(), st>>>(
+        norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+        static_cast<const double*>(coeff), static_cast<double*>(slots),
+        static_cast<double*>(rowacc), done);
+    FQEHIP_CHECK_LAUNCH("k_split_live_f64");
+    return FQEHIP_OK;
+  }
   if (split) {
     if (precision != FQEHIP_C64 || !split_supported(npair, lena, lenb)) {
       set_error("the split apply needs complex64 and npair + 1 <= %d", kBfKPad);

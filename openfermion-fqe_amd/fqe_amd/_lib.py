@@ -39,7 +39,7 @@ class Sector(ctypes.Structure):
                 ("fwd_a", c_void_p), ("fwd_b", c_void_p),
                 ("slot_a", c_void_p), ("nslot", c_int),
                 ("hbb_hi", c_void_p), ("hbb_lo", c_void_p), ("ldh", c_int64),
-                ("tile", c_int)]
+                ("tile", c_int), ("hbb_f64", c_void_p)]
 
 
 class Operator(ctypes.Structure):
@@ -138,6 +138,8 @@ _SIGNATURES = {
     "fqehip_beta_gemm": (c_int, [c_int64, c_int64, c_void_p, c_void_p,
                                  c_int64, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
+    "fqehip_beta_gemm_f64": (c_int, [c_int64, c_int64, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]),
     "fqehip_apply": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
                              c_void_p, c_double, c_double, c_double, c_double,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,

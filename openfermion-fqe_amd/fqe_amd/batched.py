@@ -46,13 +46,14 @@ class SectorSpec:
 
 def _struct_for(graph: FciGraph, op: engine.PreparedOperator, prec: int,
                 variant: str):
-    saved = settings.fused, settings.split
+    saved = settings.fused, settings.split, settings.split_fp64
     try:
         settings.fused = variant != "unfused"
         settings.split = variant == "split"
+        settings.split_fp64 = variant == "split"
         s, keep = engine.sector_struct(graph, op, prec)
     finally:
-        settings.fused, settings.split = saved
+        settings.fused, settings.split, settings.split_fp64 = saved
     if variant == "rows8":
         s.tile = 8
     elif variant == "rows4":
@@ -69,6 +70,8 @@ def variants_for(op: engine.PreparedOperator, prec: int) -> List[str]:
         out = ["rows4", "rows8", "unfused"]
         if op.pair_symmetric:
             out.insert(0, "split")
+    elif op.pair_symmetric:
+        out.insert(1, "split")
     return out
 
 

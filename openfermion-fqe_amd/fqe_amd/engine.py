@@ -42,9 +42,10 @@ class PreparedOperator:
         self._hbb = {}
 
     def beta_block(self, graph):
-        """bfloat16 hi/lo split of the beta-beta block of this operator on
-        the beta strings of ``graph`` (fqehip_beta_block), cached per
-        (nbeta, device); None when the split apply does not apply."""
+        """Beta-beta block of this operator on the beta strings of
+        ``graph`` (fqehip_beta_block): (bfloat16 hi, bfloat16 lo, ldh, beta
+        forward table, FP64 columns), cached per (nbeta, device); None when
+        the split apply does not apply."""
         if not self.pair_symmetric or self.a is None or self.complex_op:
             return None
         lenb = graph.lenb()
@@ -62,8 +63,7 @@ class PreparedOperator:
             _lib.call("fqehip_beta_block", self.npair, lenb, _lib.ptr(fwb),
                       _lib.ptr(self.a), _lib.ptr(hcol), _lib.ptr(hhi),
                       _lib.ptr(hlo), ldh, _lib.stream_ptr())
-            self._hbb[key] = (hhi, hlo, ldh, fwb)
-            del hcol
+            self._hbb[key] = (hhi, hlo, ldh, fwb, hcol)
         return self._hbb[key]
 
     def c_struct(self) -> _lib.Operator:
@@ -148,9 +148,10 @@ def sector_struct(graph, op: PreparedOperator,
                   precision: Optional[int] = None
                   ) -> Tuple[_lib.Sector, tuple]:
     """C descriptor of a sector for the pair basis of ``op``; the second
-    element keeps the tables referenced.  For complex64 and a
-    pair-symmetric operator the beta-beta block is attached (split apply,
-    ``settings.split``)."""
+    element keeps the tables referenced.  For a pair-symmetric operator the
+    beta-beta block is attached (split apply): its bfloat16 hi/lo split for
+    complex64 (``settings.split``), its FP64 columns for complex128
+    (``settings.split_fp64``)."""
     ta = graph.tables(True).pair_table(op.compressed)
     tb = graph.tables(False).pair_table(op.compressed)
     keep = [ta, tb]
@@ -160,9 +161,15 @@ def sector_struct(graph, op: PreparedOperator,
         fwb, _ = graph.tables(False).fused_tables(op.compressed)
         keep += [fwa, fwb, sla]
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
-    hbb = None
-    if (fwa is not None and settings.split and precision == _lib.C64):
+    hbb = hbb64 = None
+    if fwa is not None and settings.split and precision == _lib.C64:
         hbb = op.beta_block(graph)
+    # FP64 split kernel limits: gather depth npair + 1 <= 112, at most 64
+    # live pair rows (norb^2/4 + 1)
+    if (fwa is not None and settings.split_fp64 and precision == _lib.C128
+            and op.npair + 1 <= 112 and graph.norb() ** 2 // 4 + 1 <= 64):
+        blk = op.beta_block(graph)
+        hbb64 = blk[4] if blk else None
     s = _lib.Sector(graph.norb(), graph.nalpha(), graph.nbeta(), op.npair,
                     graph.lena(), graph.lenb(), _lib.ptr(ta), _lib.ptr(tb),
                     _lib.ptr(fwa), _lib.ptr(fwb), _lib.ptr(sla), nslot,
@@ -172,6 +179,9 @@ def sector_struct(graph, op: PreparedOperator,
     if hbb:
         keep += [hbb[0], hbb[1]]
         s.tile = int(settings.split_tile)
+    if hbb64 is not None:
+        keep.append(hbb64)
+        s.hbb_f64 = _lib.ptr(hbb64)
     return s, tuple(keep)
 
 

@@ -23,6 +23,11 @@ fused = True
 # for other operators, the row kernel also scatters beta excitations.
 split = True
 
+# complex128 applies of a pair-symmetric real operator: beta-beta block as
+# one FP64 matrix-core GEMM and an FP64 row kernel over the live alpha pair
+# rows.  Off, the FP64 row kernel also scatters beta excitations.
+split_fp64 = True
+
 # Kernel of the split row pass: 0 = MFMAs over the live pair rows of each
 # alpha string only (excitations to another string plus the folded
 # diagonal row); 16 = MFMAs over all npair + 1 pair rows.

@@ -72,23 +72,32 @@ def test_batched_graph_matches_oracle_and_sequential(fqe, dtype):
     bt.close()
 
 
-def test_every_variant_matches_oracle(fqe):
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_every_variant_matches_oracle(fqe, dtype):
+    """Each kernel variant of each precision (complex128: rows4, split,
+    unfused) in a one-sector graph against the oracle."""
     import torch
     from fqe_amd import batched
+    tdt = getattr(torch, dtype)
+    tol = MIXED_TOL if dtype == "complex64" else 1e-11
     specs = batched.random_specs(3, seed=5, norb_range=(7, 8))
     for sp in specs:
         ref, _ = _oracle(sp, 0.1, 0)
-        base = batched._Prepared(sp, torch.complex64, None)
-        for var in batched.variants_for(base.op, base.prec):
-            bt = batched.BatchedTaylor([sp], 0.1, nterms=0,
-                                       dtype=torch.complex64,
+        base = batched._Prepared(sp, tdt, None)
+        variants = batched.variants_for(base.op, base.prec)
+        if dtype == "complex128" and base.op.pair_symmetric:
+            assert "split" in variants
+        for var in variants:
+            bt = batched.BatchedTaylor([sp], 0.1, nterms=0, dtype=tdt,
                                        tiles={f"{sp.norb},{sp.nalpha},{sp.nbeta}": var},
                                        nstreams=1)
             assert bt.sectors[0].variant == var
+            assert bool(bt.sectors[0].s.hbb_f64) == (
+                var == "split" and dtype == "complex128")
             bt.build()
             bt.launch()
             ap, ev = bt.results()[0]
-            assert rel(ap, ref) < MIXED_TOL, var
+            assert rel(ap, ref) < tol, var
             assert rel(ev, sp.coeff) < 1e-7, var
             bt.close()
 

@@ -501,6 +501,63 @@ def test_split_apply(fqe, na, nb, norb):
     assert rel(split, rows) < MIXED_TOL
 
 
+@pytest.mark.parametrize("lena,lenb", [(1, 1), (5, 130), (129, 127),
+                                       (300, 257), (1000, 1000)])
+def test_beta_gemm_f64_matches_numpy(fqe, lena, lenb):
+    """fqehip_beta_gemm_f64: out = C hcol (hcol[Ib][Ib'] = Hbb[Ib'][Ib]) on
+    the FP64 matrix cores, for sizes off the 128-tile and 16-step grid."""
+    import torch
+    from fqe_amd import _lib
+    print(f"beta gemm f64 {lena}x{lenb}", flush=True)
+    rng = numpy.random.default_rng(lena * 3 + lenb)
+    c = rng.normal(size=(lena, lenb)) + 1j * rng.normal(size=(lena, lenb))
+    h = rng.normal(size=(lenb, lenb))
+    dev = _lib.device()
+    cd = torch.from_numpy(c).to(dev)
+    hd = torch.from_numpy(h).to(dev)
+    out = torch.full((lena, lenb), 5.0 + 5.0j, dtype=torch.complex128,
+                     device=dev)
+    _lib.call("fqehip_beta_gemm_f64", lena, lenb, _lib.ptr(hd), _lib.ptr(cd),
+              _lib.ptr(out), None, _lib.stream_ptr())
+    ref = c @ h
+    assert numpy.max(numpy.abs(out.cpu().numpy() - ref)) < 1e-12 * numpy.abs(ref).max()
+
+
+@pytest.mark.parametrize("na,nb,norb", [(3, 0, 7), (0, 3, 7), (5, 5, 10),
+                                        (7, 3, 10), (1, 1, 3), (4, 4, 9),
+                                        (2, 6, 8), (6, 5, 12), (7, 7, 14)])
+def test_split_apply_fp64(fqe, na, nb, norb):
+    """The complex128 split apply (FP64 beta-beta GEMM + FP64 row kernel
+    over the live alpha pair rows) is taken for a pair-symmetric operator
+    and agrees with the oracle (sectors the oracle finishes quickly) and
+    with the FP64 row kernel that scatters beta excitations itself."""
+    import ctypes
+    from fqe_amd import _lib, engine, settings
+    print(f"split apply fp64 {na} {nb} {norb}", flush=True)
+    rng = numpy.random.default_rng(norb * 11 + na * 7 + nb)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    ref = fo.apply_12(c, h1, h2, g) if g.lena * g.lenb <= 200000 else None
+    d = fqe.FqeData(na, nb, norb, dtype=numpy.complex128)
+    d.coeff = c
+    op = engine.prepare_operator((h1, h2), norb, _lib.device())
+    s, keep = engine.sector_struct(d.get_fcigraph(), op, _lib.C128)
+    assert s.hbb_f64 and not s.hbb_hi
+    assert _lib.lib().fqehip_fused_supported(_lib.C128, ctypes.byref(s),
+                                             ctypes.byref(op.c_struct()))
+    split = d.apply((h1, h2)).coeff
+    saved = settings.split_fp64
+    settings.split_fp64 = False
+    try:
+        rows = d.apply((h1, h2)).coeff
+    finally:
+        settings.split_fp64 = saved
+    assert rel(split, rows) < FP64_TOL
+    if ref is not None:
+        assert rel(split, ref) < FP64_TOL
+
+
 @pytest.mark.parametrize("tile", [16])
 @pytest.mark.parametrize("na,nb,norb", [(3, 0, 7), (5, 5, 10), (1, 1, 3),
                                         (2, 6, 8), (6, 5, 12), (7, 7, 14),
