@@ -464,10 +464,13 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
 // fragments of the live rows are rebuilt in registers per alpha row, so
 // the chunk loop runs half the MFMAs of the all-rows kernel, and each
 // lane's scatter targets (slot base and sign of its 4 output rows) are
-// computed once per alpha row and kept in registers.
+// computed once per alpha row and kept in registers.  TBC beta columns per
+// chunk; ROWLDS stages row ia of C in LDS for the beta half of the gather
+// (random 8-byte reads within the row: from LDS instead of one cache line
+// per lane through L1/L2).
 constexpr int kLiveRows = 64;               // 4 tiles of 16 rows
 
-template <int NW, int WPE>
+template <int NW, int WPE, int TBC, bool ROWLDS>
 __global__ void __launch_bounds__(NW * 64, WPE)
 k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
              const double* __restrict__ amat,
@@ -478,7 +481,9 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
              const int32_t* __restrict__ done) {
   constexpr int kT = NW * 64;
   constexpr int kRT = kLiveRows / 16 / NW;     // live row tiles per wave
-  constexpr int kGroups = kT / kBfTbC;         // gather pair groups of 8
+  constexpr int kGroups = kT / TBC;            // gather pair groups of 8
+  constexpr int kCols = 2 * TBC;
+  constexpr int kHalves = TBC / 16;
   constexpr int kGIter = kBfKPad / (8 * kGroups);
   static_assert(kRT >= 1 && kRT * NW * 16 == kLiveRows, "row tiles");
   static_assert(kGIter >= 1 && kGIter * 8 * kGroups == kBfKPad, "gather");
@@ -490,18 +495,19 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
   const int K = npair;
   const int Kg = K + 1;                        // gathered rows: pairs + C
   __bf16* dhi = reinterpret_cast<__bf16*>(smem);
-  __bf16* dlo = dhi + kBfCols * kBfStride;
-  double* drow = reinterpret_cast<double*>(dlo + kBfCols * kBfStride);
+  __bf16* dlo = dhi + kCols * kBfStride;
+  double* drow = reinterpret_cast<double*>(dlo + kCols * kBfStride);
   RowTables rt;
   rt.sbase = reinterpret_cast<int64_t*>(drow + kBfKPad);
   rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
   rt.asgn = rt.aoff + kRowTabs;
   rt.ssgn = rt.asgn + kRowTabs;
   int32_t* lpair = rt.ssgn + kRowTabs;         // [kLiveRows] pair of live row
+  float2* crow = reinterpret_cast<float2*>(lpair + kLiveRows);  // ROWLDS
 
-  const int t = tid % kBfTbC;                  // gather column
-  const int g = tid / kBfTbC;                  // gather pair group
-  const int nchunks = static_cast<int>((lenb + kBfTbC - 1) / kBfTbC);
+  const int t = tid % TBC;                     // gather column
+  const int g = tid / TBC;                     // gather pair group
+  const int nchunks = static_cast<int>((lenb + TBC - 1) / TBC);
   const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(float));
   const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
 
@@ -534,6 +540,10 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
                      : amat[static_cast<int64_t>(npair) * K + dp];
       }
       drow[k] = v;
+    }
+    if (ROWLDS) {
+      const float2* src = reinterpret_cast<const float2*>(coeff) + ia * lenb;
+      for (int64_t x = tid; x < lenb; x += kT) crow[x] = src[x];
     }
     __syncthreads();
     // A' fragments of the live rows (bf16 hi / lo split), the scatter
@@ -584,7 +594,7 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
     const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(float));
     float* racc = rowacc + ia * lenb * 2;
     for (int chunk = 0; chunk < nchunks; ++chunk) {
-      const int64_t ib0 = static_cast<int64_t>(chunk) * kBfTbC;
+      const int64_t ib0 = static_cast<int64_t>(chunk) * TBC;
       {
         const int ib = static_cast<int>(ib0) + t;
         const bool colok = ib < lenb;
@@ -604,7 +614,8 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
           for (int j = 0; j < 8; ++j) {
             const int32_t off = aof[it][j];
             ca[j] = bload_f2(rc, (colok && off >= 0) ? (off + 2 * ib) * 4 : -1);
-            cb[j] = bload_f2(rc, bt[j] ? crow_b + 8 * (iabs(bt[j]) - 1) : -1);
+            if (ROWLDS) cb[j] = crow[bt[j] ? iabs(bt[j]) - 1 : 0];
+            else cb[j] = bload_f2(rc, bt[j] ? crow_b + 8 * (iabs(bt[j]) - 1) : -1);
           }
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
@@ -619,13 +630,13 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
           }
           *reinterpret_cast<bf16x8*>(dhi + t * kBfStride + q0) = hr;
           *reinterpret_cast<bf16x8*>(dlo + t * kBfStride + q0) = lr;
-          *reinterpret_cast<bf16x8*>(dhi + (kBfTbC + t) * kBfStride + q0) = hi;
-          *reinterpret_cast<bf16x8*>(dlo + (kBfTbC + t) * kBfStride + q0) = li;
+          *reinterpret_cast<bf16x8*>(dhi + (TBC + t) * kBfStride + q0) = hi;
+          *reinterpret_cast<bf16x8*>(dlo + (TBC + t) * kBfStride + q0) = li;
         }
       }
       __syncthreads();
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < kHalves; ++h) {
         floatx4 are[kRT], aim[kRT];
 #pragma unroll
         for (int r = 0; r < kRT; ++r) {
@@ -636,7 +647,7 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
         for (int kt = 0; kt < kBfKSteps; ++kt) {
           const int kq = kt * 32 + 8 * (lane >> 4);
           const int cre = h * 16 + (lane & 15);
-          const int cim = kBfTbC + cre;
+          const int cim = TBC + cre;
           const bf16x8 bhr = *reinterpret_cast<const bf16x8*>(dhi + cre * kBfStride + kq);
           const bf16x8 blr = *reinterpret_cast<const bf16x8*>(dlo + cre * kBfStride + kq);
           const bf16x8 bhi = *reinterpret_cast<const bf16x8*>(dhi + cim * kBfStride + kq);
@@ -1085,9 +1096,10 @@ size_t bf16_lds_bytes(int64_t lenb, bool split = false) {
          (split ? kBfKPad * sizeof(double) : 2 * lenb * sizeof(float));
 }
 
-size_t split_live_lds_bytes() {
-  return 2 * kBfCols * kBfStride * sizeof(__bf16) + kBfKPad * sizeof(double) +
-         row_tables_bytes() + kLiveRows * sizeof(int32_t);
+size_t split_live_lds_bytes(int tbc, bool rowlds, int64_t lenb) {
+  return 2 * (2 * tbc) * kBfStride * sizeof(__bf16) + kBfKPad * sizeof(double) +
+         row_tables_bytes() + kLiveRows * sizeof(int32_t) +
+         (rowlds ? lenb * 8 : 0);
 }
 
 size_t split_live_f64_lds_bytes() {
@@ -1169,10 +1181,21 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
         set_error("the live-rows kernel needs norb^2/4 + 1 <= %d", kLiveRows);
         return FQEHIP_ERR_INVALID;
       }
-      k_split_live<4, 3><<<fused_grid(lena), 256, split_live_lds_bytes(), st>>>(
-          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
-          static_cast<const float*>(coeff), static_cast<float*>(slots),
-          static_cast<float*>(rowacc), done);
+      // 16 columns per chunk with row ia in LDS while three workgroups
+      // still fit a CU's LDS, else 32 columns and the row from L2
+      const size_t lds16 = split_live_lds_bytes(16, true, lenb);
+      if (lds16 <= kLdsLimit / 3) {
+        k_split_live<4, 3, 16, true><<<fused_grid(lena), 256, lds16, st>>>(
+            norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+            static_cast<const float*>(coeff), static_cast<float*>(slots),
+            static_cast<float*>(rowacc), done);
+      } else {
+        k_split_live<4, 3, 32, false>
+            <<<fused_grid(lena), 256, split_live_lds_bytes(32, false, lenb), st>>>(
+                norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+                static_cast<const float*>(coeff), static_cast<float*>(slots),
+                static_cast<float*>(rowacc), done);
+      }
       FQEHIP_CHECK_LAUNCH("k_split_live");
       return FQEHIP_OK;
     }
