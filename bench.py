@@ -56,6 +56,9 @@ def parse():
                     help="kernel of the complex64 split row pass (0: live "
                          "pair rows, 16: all pair rows; default: the engine "
                          "setting)")
+    ap.add_argument("--split-tile-fp64", type=int, default=None,
+                    help="kernel of the complex128 split row pass (0: "
+                         "column-owned waves, 32: shared D tile)")
     ap.add_argument("--no-split-fp64", action="store_true",
                     help="FP64 engine without the split apply (the FP64 row "
                          "kernel also scatters beta excitations)")
@@ -484,6 +487,8 @@ def main():
         settings.split_tile = args.split_tile
     if args.no_split_fp64:
         settings.split_fp64 = False
+    if args.split_tile_fp64 is not None:
+        settings.split_tile_fp64 = args.split_tile_fp64
     if args.copy_probe:
         src = torch.empty(args.copy_probe, dtype=torch.uint8, device="cuda")
         dst = torch.empty_like(src)

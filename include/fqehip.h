@@ -179,7 +179,9 @@ typedef struct fqehip_sector {
    * (4 or 8 waves per alpha row); on the split path 0 selects the kernel
    * over the live pair rows of each alpha string (excitations to another
    * string, plus the folded diagonal row), FQEHIP_TILE_SPLIT_ALLROWS the
-   * kernel over all npair + 1 rows. */
+   * kernel over all npair + 1 rows; on the complex128 split path 0 selects
+   * the column-owned FP64 kernel, FQEHIP_TILE_SPLIT_F64_DTILE the one with
+   * a shared D tile. */
   int tile;
   /* FP64 beta-beta block of the same operator, column-major [lenb][lenb]
    * (entry [Ib][Ib'] = Hbb[Ib'][Ib], the hcol output of fqehip_beta_block).
@@ -194,6 +196,9 @@ typedef struct fqehip_sector {
 /* split path (hbb set): MFMAs over all npair + 1 pair rows instead of the
  * live rows of each alpha string (the default, tile 0) */
 #define FQEHIP_TILE_SPLIT_ALLROWS 16
+/* complex128 split path (hbb_f64 set): the row kernel with a shared,
+ * double-buffered D tile instead of column-owned waves (the default) */
+#define FQEHIP_TILE_SPLIT_F64_DTILE 32
 
 typedef struct fqehip_operator {
   int complex_op;   /* 0: A is (npair+1) x npair; 1: 2(npair+1) x 2npair */

@@ -588,7 +588,8 @@ int check_sector_op(int precision, const fqehip_sector* s,
                    "complex_op=%d", op->complex_op);
   FQEHIP_CHECK_ARG(s->tile == 0 || s->tile == FQEHIP_TILE_ROWS4 ||
                        s->tile == FQEHIP_TILE_ROWS8 ||
-                       s->tile == FQEHIP_TILE_SPLIT_ALLROWS, "tile=%d", s->tile);
+                       s->tile == FQEHIP_TILE_SPLIT_ALLROWS ||
+                       s->tile == FQEHIP_TILE_SPLIT_F64_DTILE, "tile=%d", s->tile);
   return FQEHIP_OK;
 }
 

@@ -1030,6 +1030,183 @@ k_split_live_f64(int norb, int compressed, int npair, int64_t lena,
   }
 }
 
+// ---------------- FP64 split kernel, column-owned waves ---------------------
+// Same product as k_split_live_f64, organised without a shared D tile: the
+// live rows' A' fragments sit in LDS (written once per alpha string), and
+// each wave owns 16 columns of a 64-column chunk.  A wave gathers the D'
+// values of its columns straight into its MFMA B-operand registers
+// (lane l: k = 4 kt + l/16, column l%16), then runs all four 16-row tiles
+// of live rows over them.  The chunk loop has no barrier, so one wave's
+// gather overlaps another wave's MFMAs.
+constexpr int kCF64Threads = 256;
+constexpr int kCF64Cols = 64;               // columns per chunk (4 waves x 16)
+constexpr int kCF64AS = kLiveRows + 4;      // doubles per LDS A' k-row (padding)
+constexpr int kCF64Batch = 4;               // k steps gathered per load batch
+constexpr int kCF64Parts = 4;               // k-step parts held in registers
+
+__global__ void __launch_bounds__(kCF64Threads, 2)
+k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
+                 int64_t lenb, const double* __restrict__ amat,
+                 const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+                 const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
+                 int nslot, const double* __restrict__ coeff,
+                 double* __restrict__ slots, double* __restrict__ rowacc,
+                 const int32_t* __restrict__ done) {
+  static_assert(kF64KSteps % kCF64Parts == 0, "parts of the k steps");
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int Kg = K + 1;                     // gathered rows: pairs + C
+  const int ksteps = (Kg + 3) / 4;
+  double* alds = reinterpret_cast<double*>(smem);     // [kF64KMax][kCF64AS]
+  double* drow = alds + kF64KMax * kCF64AS;
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(drow + kF64KPad);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  int32_t* lpair = rt.ssgn + kRowTabs;      // [kLiveRows] pair of live row
+
+  const int nchunks = static_cast<int>((lenb + kCF64Cols - 1) / kCF64Cols);
+  const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(double));
+  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+  const int kq = lane >> 4;                 // k within a k step
+  const int cl = lane & 15;                 // column within the wave's 16
+
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+    load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                            kCF64Threads, K);
+    __syncthreads();
+    if (wave == 0) {
+      int base = 0;
+      for (int p0 = 0; p0 < K; p0 += 64) {
+        const int p = p0 + lane;
+        const bool live = p < K && rt.sbase[p] >= 0;
+        const uint64_t m = __ballot(live);
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (live && pos < kLiveRows) lpair[pos] = p;
+        base += __popcll(m);
+      }
+      if (lane >= base) lpair[lane] = lane == base ? npair : -1;
+    }
+    for (int k = tid; k < Kg; k += kCF64Threads) {
+      double v = 0.0;
+      for (int i = 0; i < norb; ++i) {
+        const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
+        if (rt.sbase[dp] == -2)
+          v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
+                     : amat[static_cast<int64_t>(npair) * K + dp];
+      }
+      drow[k] = v;
+    }
+    __syncthreads();
+    // A'[live row m][k] into LDS as [k][m]
+    for (int x = tid; x < kF64KMax * kLiveRows; x += kCF64Threads) {
+      const int k = x / kLiveRows;
+      const int m = x - k * kLiveRows;
+      const int p = lpair[m];
+      double v = 0.0;
+      if (p == npair) {
+        if (k < Kg) v = drow[k];
+      } else if (p >= 0) {
+        if (k < K) v = amat[static_cast<int64_t>(p) * K + k];
+        else if (k == K) v = amat[static_cast<int64_t>(npair) * K + p];
+      }
+      alds[k * kCF64AS + m] = v;
+    }
+    __syncthreads();
+    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(double));
+    double* racc = rowacc + ia * lenb * 2;
+
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
+      const int64_t jb = static_cast<int64_t>(chunk) * kCF64Cols + wave * 16 + cl;
+      const bool colok = jb < lenb;
+      const int ib = static_cast<int>(colok ? jb : 0);
+      // the k steps in kCF64Parts parts: B fragments D'[4 kt + kq][column]
+      // of one part in registers, accumulators of all four row tiles kept
+      double4_t accr[kLiveRows / 16], acci[kLiveRows / 16];
+#pragma unroll
+      for (int r = 0; r < kLiveRows / 16; ++r) {
+        accr[r] = double4_t{0, 0, 0, 0};
+        acci[r] = double4_t{0, 0, 0, 0};
+      }
+#pragma unroll 1
+      for (int hk = 0; hk < kCF64Parts; ++hk) {
+        constexpr int kH = kF64KSteps / kCF64Parts;
+        double bre[kH], bim[kH];
+#pragma unroll
+        for (int b0 = 0; b0 < kH; b0 += kCF64Batch) {
+          int32_t bt[kCF64Batch], as[kCF64Batch];
+          double2 ca[kCF64Batch], cb[kCF64Batch];
+#pragma unroll
+          for (int j = 0; j < kCF64Batch; ++j) {
+            const int q = 4 * (hk * kH + b0 + j) + kq;
+            bt[j] = (b0 + j < kH)
+                        ? bload_i32(rpb, (colok && q < K)
+                                             ? (q * static_cast<int>(lenb) + ib) * 4 : -1)
+                        : 0;
+          }
+#pragma unroll
+          for (int j = 0; j < kCF64Batch; ++j) {
+            const int q = 4 * (hk * kH + b0 + j) + kq;
+            const int32_t off = (colok && b0 + j < kH && q < Kg) ? rt.aoff[q] : -1;
+            as[j] = off >= 0 ? rt.asgn[q] : 0;
+            ca[j] = bload_d2(rc, off >= 0 ? (off + 2 * ib) * 8 : -1);
+            cb[j] = bload_d2(rc, bt[j] ? crow_b + 16 * (iabs(bt[j]) - 1) : -1);
+          }
+#pragma unroll
+          for (int j = 0; j < kCF64Batch; ++j) {
+            if (b0 + j < kH) {
+              const double sa = static_cast<double>(as[j]);
+              const double sb = bt[j] > 0 ? 2.0 : (bt[j] < 0 ? -2.0 : 0.0);
+              bre[b0 + j] = sa * ca[j].x + sb * cb[j].x;
+              bim[b0 + j] = sa * ca[j].y + sb * cb[j].y;
+            }
+          }
+        }
+        // A from LDS: A[r = l%16][k = l/16]
+#pragma unroll
+        for (int r = 0; r < kLiveRows / 16; ++r) {
+          const double* ab = alds + (hk * kH * 4 + kq) * kCF64AS + r * 16 + cl;
+#pragma unroll
+          for (int kt = 0; kt < kH; ++kt) {
+            if (hk * kH + kt < ksteps) {
+              const double av = ab[kt * 4 * kCF64AS];
+              accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bre[kt], accr[r], 0, 0, 0);
+              acci[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bim[kt], acci[r], 0, 0, 0);
+            }
+          }
+        }
+      }
+      // f64 MFMA layout: item i of lane l is row l/16 + 4 i, col l%16
+      if (colok) {
+#pragma unroll
+        for (int r = 0; r < kLiveRows / 16; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int po = lpair[r * 16 + kq + 4 * i];
+            if (po == npair) {
+              double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
+              double2 o = *d;
+              o.x += accr[r][i];
+              o.y += acci[r][i];
+              *d = o;
+            } else if (po >= 0) {
+              const int64_t sb = rt.sbase[po];
+              const double sg = static_cast<double>(rt.ssgn[po]);
+              *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
+                  make_double2(sg * accr[r][i], sg * acci[r][i]);
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------------------
 // slot reduction + epilogue
 // ---------------------------------------------------------------------------
@@ -1125,6 +1302,11 @@ size_t split_live_f64_lds_bytes() {
          kLiveRows * sizeof(int32_t);
 }
 
+size_t split_cols_f64_lds_bytes() {
+  return (kF64KMax * kCF64AS + kF64KPad) * sizeof(double) + row_tables_bytes() +
+         kLiveRows * sizeof(int32_t);
+}
+
 size_t f64_lds_bytes(int64_t lenb) {
   return 2 * kF64Tile * sizeof(double) + row_tables_bytes() +
          2 * lenb * sizeof(double);
@@ -1180,11 +1362,19 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
                 kF64KMax, kLiveRows);
       return FQEHIP_ERR_INVALID;
     }
-    k_split_live_f64<<<fused_grid(lena), kSF64Threads, split_live_f64_lds_bytes(), st>>>(
+    if (tile == FQEHIP_TILE_SPLIT_F64_DTILE) {
+      k_split_live_f64<<<fused_grid(lena), kSF64Threads, split_live_f64_lds_bytes(), st>>>(
+          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+          static_cast<const double*>(coeff), static_cast<double*>(slots),
+          static_cast<double*>(rowacc), done);
+      FQEHIP_CHECK_LAUNCH("k_split_live_f64");
+      return FQEHIP_OK;
+    }
+    k_split_cols_f64<<<fused_grid(lena), kCF64Threads, split_cols_f64_lds_bytes(), st>>>(
         norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
         static_cast<const double*>(coeff), static_cast<double*>(slots),
         static_cast<double*>(rowacc), done);
-    FQEHIP_CHECK_LAUNCH("k_split_live_f64");
+    FQEHIP_CHECK_LAUNCH("k_split_cols_f64");
     return FQEHIP_OK;
   }
   if (split) {

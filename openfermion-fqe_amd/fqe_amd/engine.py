@@ -182,6 +182,7 @@ def sector_struct(graph, op: PreparedOperator,
     if hbb64 is not None:
         keep.append(hbb64)
         s.hbb_f64 = _lib.ptr(hbb64)
+        s.tile = int(settings.split_tile_fp64)
     return s, tuple(keep)
 
 

@@ -32,3 +32,8 @@ split_fp64 = True
 # alpha string only (excitations to another string plus the folded
 # diagonal row); 16 = MFMAs over all npair + 1 pair rows.
 split_tile = 0
+
+# Kernel of the complex128 split row pass: 0 = column-owned waves (the D'
+# values of a wave's columns gathered into its MFMA registers, A' in LDS);
+# 32 = the kernel with a shared, double-buffered D tile.
+split_tile_fp64 = 0

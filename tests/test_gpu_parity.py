@@ -556,6 +556,22 @@ def test_split_apply_fp64(fqe, na, nb, norb):
     assert rel(split, rows) < FP64_TOL
     if ref is not None:
         assert rel(split, ref) < FP64_TOL
+    # the column-owned kernel (default) and the shared-D-tile kernel
+    # accumulate the same products in the same order: identical bits for
+    # one prepared operator (each preparation rebuilds H_bb with LDS
+    # atomics, which may differ in the last bit)
+    outs = []
+    saved = settings.split_tile_fp64
+    try:
+        for tile in (0, 32):
+            settings.split_tile_fp64 = tile
+            e = copy.deepcopy(d)
+            e.apply_inplace_scaled((h1, h2), op=op)
+            outs.append(e.coeff)
+    finally:
+        settings.split_tile_fp64 = saved
+    assert rel(outs[0], split) < FP64_TOL
+    assert numpy.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("tile", [16])
@@ -569,7 +585,7 @@ def test_split_apply_live_rows(fqe, na, nb, norb, tile):
     agree with the oracle (sectors small enough for the oracle; the full-size sectors,
     norb 14 and 15 at half filling, are checked against the all-rows
     kernel only)."""
-    from fqe_amd import settings
+    from fqe_amd import _lib, engine, settings
     print(f"split live rows {tile}: {na} {nb} {norb}", flush=True)
     rng = numpy.random.default_rng(norb * 17 + na * 3 + nb)
     h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
@@ -578,13 +594,20 @@ def test_split_apply_live_rows(fqe, na, nb, norb, tile):
     ref = fo.apply_12(c, h1, h2, g) if g.lena * g.lenb <= 200000 else None
     d = fqe.FqeData(na, nb, norb, dtype=numpy.complex64)
     d.coeff = c
-    out = d.apply((h1, h2)).coeff
+    # one prepared operator for both (each preparation rebuilds H_bb with
+    # LDS atomics, which may differ in the last bit)
+    op = engine.prepare_operator((h1, h2), norb, _lib.device())
+    outs = []
     saved = settings.split_tile
-    settings.split_tile = tile
     try:
-        base = d.apply((h1, h2)).coeff
+        for t in (0, tile):
+            settings.split_tile = t
+            e = copy.deepcopy(d)
+            e.apply_inplace_scaled((h1, h2), op=op)
+            outs.append(e.coeff)
     finally:
         settings.split_tile = saved
+    out, base = outs
     if ref is not None:
         assert rel(out, ref) < MIXED_TOL
     assert numpy.array_equal(out, base)
