@@ -1171,12 +1171,16 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
 #pragma unroll
         for (int r = 0; r < kLiveRows / 16; ++r) {
           const double* ab = alds + (hk * kH * 4 + kq) * kCF64AS + r * 16 + cl;
+          // all of the part's A values first: the LDS reads are in flight
+          // together instead of one wait per MFMA pair
+          double av[kH];
+#pragma unroll
+          for (int kt = 0; kt < kH; ++kt) av[kt] = ab[kt * 4 * kCF64AS];
 #pragma unroll
           for (int kt = 0; kt < kH; ++kt) {
             if (hk * kH + kt < ksteps) {
-              const double av = ab[kt * 4 * kCF64AS];
-              accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bre[kt], accr[r], 0, 0, 0);
-              acci[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bim[kt], acci[r], 0, 0, 0);
+              accr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kt], bre[kt], accr[r], 0, 0, 0);
+              acci[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[kt], bim[kt], acci[r], 0, 0, 0);
             }
           }
         }
