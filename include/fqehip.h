@@ -231,12 +231,14 @@ int fqehip_fused_supported(int precision, const fqehip_sector* s,
  * fqehip_operator, A[p][q] = A[q][p] for p, q < npair):
  *   Hbb = sum_pq A[p][q] E^b_p E^b_q + sum_q A[npair][q] E^b_q
  * on the lenb beta strings, from the beta forward table d_fwd_b.  Writes
- * d_hcol[Ib][Ib'] = Hbb[Ib'][Ib] (FP64, lenb x lenb, scratch the caller
- * may free afterwards) and the bfloat16 hi + lo split of Hbb, row-major
+ * d_hcol[Ib][Ib'] = Hbb[Ib'][Ib] (FP64, lenb x lenb: the hbb_f64 input of
+ * the complex128 split apply, else scratch) and the bfloat16 hi + lo split
+ * of Hbb (the complex64 split apply's input), row-major
  * [ldh][ldh] zero-padded, ldh = fqehip_beta_block_ld(lenb).  It is the
  * same-spin part of the operator that the reference's lm kernels apply
  * per spin (src/fqe/lib/fqe_data.c:668-720); here it is a matrix so that
- * the beta half of every apply is one GEMM.  lenb <= 20480. */
+ * the beta half of every apply is one GEMM.  (lenb + npair) * 8 bytes
+ * <= 160 KiB; the same bits on every run. */
 int64_t fqehip_beta_block_ld(int64_t lenb);
 int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b,
                       const double* d_a, double* d_hcol, uint16_t* d_hhi,

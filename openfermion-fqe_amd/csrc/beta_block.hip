@@ -43,34 +43,50 @@ __device__ __forceinline__ float from_bf16(__bf16 h) {
 
 __device__ __forceinline__ int32_t iabs(int32_t v) { return v < 0 ? -v : v; }
 
-// Column Ib of Hbb: LDS accumulator of lenb doubles.  Work items (q, p):
-// p = -1 is the one-body term of q, p >= 0 the two-body product.
+// Column Ib of Hbb: LDS accumulator of lenb doubles, built in a fixed
+// order so that every run gives the same bits.  For each q in turn the
+// intermediate string s = E_q Ib is uniform over the workgroup; the
+// off-diagonal pairs p move one electron of s, so they reach distinct
+// targets t != s and are added by one thread each; the diagonal pairs
+// (t = s) and the one-body term of q are summed by wave 0 in a fixed lane
+// order (per-lane strided sums, then an xor butterfly) and added to s.
 __global__ void __launch_bounds__(kBlockThreads)
 k_beta_block(int npair, int64_t lenb, const int32_t* __restrict__ fwd,
              const double* __restrict__ amat, double* __restrict__ hcol) {
   extern __shared__ double col[];
+  double* diag = col + lenb;                  // [npair] diagonal terms of q
   const int64_t ib = blockIdx.x;
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
   for (int64_t x = tid; x < lenb; x += kBlockThreads) col[x] = 0.0;
   __syncthreads();
   const int K = npair;
-  const int items = npair * (npair + 1);
-  for (int c = tid; c < items; c += kBlockThreads) {
-    const int q = c / (npair + 1);
-    const int p = c - q * (npair + 1) - 1;
+  for (int q = 0; q < K; ++q) {
     const int32_t f = fwd[static_cast<int64_t>(q) * lenb + ib];
-    if (!f) continue;
-    const int64_t t = iabs(f) - 1;
-    if (p < 0) {
-      atomicAdd(&col[t], (f > 0 ? 1.0 : -1.0) * amat[static_cast<int64_t>(npair) * K + q]);
-      continue;
+    if (!f) continue;                         // uniform over the workgroup
+    const int64_t s = iabs(f) - 1;
+    const double sf = f > 0 ? 1.0 : -1.0;
+    for (int p = tid; p < K; p += kBlockThreads) {
+      const int32_t g = fwd[static_cast<int64_t>(p) * lenb + s];
+      double v = 0.0;
+      if (g) {
+        const double c = (g > 0 ? sf : -sf) * amat[static_cast<int64_t>(p) * K + q];
+        const int64_t t = iabs(g) - 1;
+        if (t == s) v = c;
+        else col[t] += c;
+      }
+      diag[p] = v;
     }
-    const int32_t g = fwd[static_cast<int64_t>(p) * lenb + t];
-    if (!g) continue;
-    const double s = ((f > 0) == (g > 0)) ? 1.0 : -1.0;
-    atomicAdd(&col[iabs(g) - 1], s * amat[static_cast<int64_t>(p) * K + q]);
+    __syncthreads();
+    if (tid < 64) {
+      double v = 0.0;
+      for (int p = lane; p < K; p += 64) v += diag[p];
+#pragma unroll
+      for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+      if (lane == 0) col[s] += v + sf * amat[static_cast<int64_t>(npair) * K + q];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   double* dst = hcol + ib * lenb;
   for (int64_t x = tid; x < lenb; x += kBlockThreads) dst[x] = col[x];
 }
@@ -456,7 +472,7 @@ extern "C" int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b
   FQEHIP_CHECK_ARG(ldh == fqehip_beta_block_ld(lenb),
                    "ldh=%lld, expected %lld", static_cast<long long>(ldh),
                    static_cast<long long>(fqehip_beta_block_ld(lenb)));
-  FQEHIP_CHECK_ARG(lenb * static_cast<int64_t>(sizeof(double)) <= 160 * 1024,
+  FQEHIP_CHECK_ARG((lenb + npair) * static_cast<int64_t>(sizeof(double)) <= 160 * 1024,
                    "lenb=%lld exceeds the LDS column of the beta block",
                    static_cast<long long>(lenb));
   if (lenb == 0) return FQEHIP_OK;
@@ -465,7 +481,8 @@ extern "C" int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b
   hipStream_t st = as_stream(stream);
   // npair == 0 (no orbitals) leaves the LDS column zero: Hbb = 0
   k_beta_block<<<static_cast<unsigned>(lenb), kBlockThreads,
-                 lenb * sizeof(double), st>>>(npair, lenb, d_fwd_b, d_a, d_hcol);
+                 (lenb + npair) * sizeof(double), st>>>(npair, lenb, d_fwd_b, d_a,
+                                                        d_hcol);
   FQEHIP_CHECK_LAUNCH("k_beta_block");
   dim3 grid(static_cast<unsigned>(ldh / 32), static_cast<unsigned>(ldh / 32));
   k_beta_split<<<grid, 256, 0, st>>>(lenb, ldh, d_hcol,

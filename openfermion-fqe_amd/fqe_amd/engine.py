@@ -49,7 +49,7 @@ class PreparedOperator:
         if not self.pair_symmetric or self.a is None or self.complex_op:
             return None
         lenb = graph.lenb()
-        if lenb * 8 > 160 * 1024:
+        if (lenb + self.npair) * 8 > 160 * 1024:
             return None
         key = (graph.nbeta(), self.a.device.index)
         if key not in self._hbb:

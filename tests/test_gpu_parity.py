@@ -436,8 +436,9 @@ def _bf16_to_f64(t):
                                           (5, 5, "symmetric")])
 def test_beta_block_matches_dense_sector(fqe, nb, norb, kind):
     """Hbb equals the dense CI matrix of the (0, nb) sector, in which no
-    alpha string exists to excite (lena = 1); its bf16 hi + lo split
-    represents it to 2^-16 relative and is zero-padded."""
+    alpha string exists to excite (lena = 1), with the same bits on a
+    second build; its bf16 hi + lo split represents it to 2^-16 relative
+    and is zero-padded."""
     import torch
     from fqe_amd import _lib, engine
     print(f"beta block nb={nb} norb={norb} {kind}", flush=True)
@@ -460,6 +461,12 @@ def test_beta_block_matches_dense_sector(fqe, nb, norb, kind):
               ldh, _lib.stream_ptr())
     hbb = hcol.cpu().numpy().T
     assert rel(hbb, dense) < FP64_TOL
+    # the build runs in a fixed order: a second build gives the same bits
+    hcol2 = torch.full_like(hcol, 3.0)
+    _lib.call("fqehip_beta_block", op.npair, lenb, _lib.ptr(fwb),
+              _lib.ptr(op.a), _lib.ptr(hcol2), _lib.ptr(hhi), _lib.ptr(hlo),
+              ldh, _lib.stream_ptr())
+    assert torch.equal(hcol, hcol2)
     split = _bf16_to_f64(hhi) + _bf16_to_f64(hlo)
     assert numpy.all(split[lenb:, :] == 0) and numpy.all(split[:, lenb:] == 0)
     err = numpy.abs(split[:lenb, :lenb] - dense)
