@@ -254,6 +254,9 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
   float* accrow = reinterpret_cast<float*>(rt.ssgn + kRowTabs);
   double* drow = reinterpret_cast<double*>(rt.ssgn + kRowTabs);  // kSplit
   const int Kg = kSplit ? K + 1 : K;       // gathered rows of the D tile
+  // k rows the MFMAs read (32 per step): small sectors skip the padding
+  const int ksteps = (Kg + 31) / 32;
+  const int kpad = 32 * ksteps;
 
   bf16x8 ahi[kRT][kBfKSteps], alo[kRT][kBfKSteps];
 #pragma unroll
@@ -330,8 +333,9 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
           const int q = 8 * (g + it * kGroups) + j;
-          bt[it][j] = bload_i32(rpb, (ib < lenb && q < K)
-                                         ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+          bt[it][j] = q < kpad ? bload_i32(rpb, (ib < lenb && q < K)
+                                                    ? (q * static_cast<int>(lenb) + ib) * 4 : -1)
+                               : 0;
         }
     };
     load_bt(0);
@@ -344,6 +348,7 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
 #pragma unroll
         for (int it = 0; it < kGIter; ++it) {
           const int q0 = 8 * (g + it * kGroups);
+          if (q0 >= kpad) continue;        // rows no MFMA reads
           bf16x8 hr, lr, hi, li;
           float2 ca[8], cb[8];
           float sa[8];
@@ -398,6 +403,7 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
           }
 #pragma unroll
           for (int kt = 0; kt < kBfKSteps; ++kt) {
+            if (kt >= ksteps) break;
             const int kq = kt * 32 + 8 * (lane >> 4);
             const int cre = h * 16 + (lane & 15);
             const int cim = kBfTbC + cre;
@@ -1136,6 +1142,7 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
 #pragma unroll 1
       for (int hk = 0; hk < kCF64Parts; ++hk) {
         constexpr int kH = kF64KSteps / kCF64Parts;
+        if (hk * kH >= ksteps) break;      // parts no MFMA reads
         double bre[kH], bim[kH];
 #pragma unroll
         for (int b0 = 0; b0 < kH; b0 += kCF64Batch) {
