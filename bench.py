@@ -69,6 +69,9 @@ def parse():
                          "sectors, norb 6..12, seed 3, one apply + 10-term "
                          "Taylor each) and print its own JSON line")
     ap.add_argument("--sectors", type=int, default=256)
+    ap.add_argument("--no-group", action="store_true",
+                    help="batched: every sector on its own kernels (no "
+                         "grouped launch of the rows4/rows8 sectors)")
     ap.add_argument("--davidson", action="store_true",
                     help="run the Davidson workload instead: train the "
                          "learned preconditioner on small synthetic "
@@ -168,8 +171,10 @@ def batched_main(args):
     tiles = batched.autotune(specs, torch.complex64, reps=2)
     t_tune = time.perf_counter() - t0
     mixed = batched.BatchedTaylor(specs, 0.1, nterms=10,
-                                  dtype=torch.complex64, tiles=tiles)
+                                  dtype=torch.complex64, tiles=tiles,
+                                  group=not args.no_group)
     t_build = mixed.build()
+    ngrouped = mixed.grouped()
     for _ in range(args.warmup):
         mixed.launch()
     torch.cuda.synchronize()
@@ -212,7 +217,7 @@ def batched_main(args):
         "config": {"workload": f"{len(specs)} sectors, norb 6..12, nalpha and "
                                f"nbeta uniform in [1, norb-1], seed 3, "
                                f"{ndet} determinants in total",
-                   "variants": counts},
+                   "variants": counts, "grouped_sectors": ngrouped},
         "fp64_sequential": {"value": round(applies / wall_d, 1),
                             "unit": "sector-applies/s",
                             "ms_per_batch": round(1e3 * wall_d, 3)},

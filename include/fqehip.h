@@ -405,13 +405,19 @@ typedef struct fqehip_batch fqehip_batch;
 /* Records the batch into a graph with up to nstreams parallel branches
  * (1..32), on capture streams of the batch's own (`stream` is not used
  * while recording).  Buffers are referenced, not copied: every launch on
- * a stream reads d_coeff0 and rewrites the outputs. */
+ * a stream reads d_coeff0 and rewrites the outputs.  With group != 0 the
+ * complex64 sectors that take the fused row kernel without the split
+ * (hbb_hi NULL) run as one group: per Taylor step one row launch and one
+ * reduction cover all of them (the rows8 row body), instead of kernels per
+ * sector; the other sectors keep their own kernels on the other branches. */
 int fqehip_batch_create(int precision, int nitems,
                         const fqehip_batch_item* items, double time,
-                        int nterms, int nstreams, void* stream,
+                        int nterms, int nstreams, int group, void* stream,
                         fqehip_batch** out);
 int fqehip_batch_launch(fqehip_batch* batch, void* stream);
 int fqehip_batch_nodes(const fqehip_batch* batch);
+/* Number of sectors the batch runs as one group. */
+int fqehip_batch_grouped(const fqehip_batch* batch);
 void fqehip_batch_destroy(fqehip_batch* batch);
 
 /* ---------------------------------------------------------------------

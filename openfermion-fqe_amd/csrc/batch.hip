@@ -18,10 +18,20 @@ struct fqehip_batch {
   hipGraphExec_t exec = nullptr;
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
+  fqehip::GroupSector* d_group = nullptr;   // grouped sectors (device)
+  int ngroup = 0;
   int nodes = 0;
 };
 
 namespace fqehip {
+
+// Defined in fused.hip.
+int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
+                      int64_t max_lenb, int step, hipStream_t st);
+int launch_group_reduce(const GroupSector* d_secs, int nsec, int64_t blocks,
+                        int step, double ar, double ai, hipStream_t st);
+int group_reduce_threads();
+
 namespace {
 
 size_t esize(int precision) {
@@ -39,6 +49,7 @@ void release(fqehip_batch* b) {
   // teardown: failures here leave nothing to recover, so they are ignored
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_group) (void)hipFree(b->d_group);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   for (hipStream_t s : b->streams) (void)hipStreamDestroy(s);
   delete b;
@@ -77,6 +88,40 @@ int enqueue_item(int precision, const fqehip_batch_item& it, double time,
   return FQEHIP_OK;
 }
 
+// The grouped sectors on stream st: per sector the copy of C0 into the
+// term buffer and the FP64 initial sum, then per step one grouped row
+// launch and one grouped reduction over all of them.
+int enqueue_group(int precision, const fqehip_batch_item* items,
+                  const std::vector<int>& grouped, const GroupSector* d_secs,
+                  int64_t rows, int64_t blocks, int64_t max_lenb,
+                  bool any_apply, double time, int nterms, hipStream_t st,
+                  int* nodes) {
+  void* sv = reinterpret_cast<void*>(st);
+  for (int i : grouped) {
+    const fqehip_batch_item& it = items[i];
+    const int64_t n = it.sector.lena * it.sector.lenb;
+    if (hipMemcpyAsync(it.d_term, it.d_coeff0, n * esize(precision),
+                       hipMemcpyDeviceToDevice, st) != hipSuccess) {
+      set_error("batch: copy of the initial term failed");
+      return FQEHIP_ERR_HIP;
+    }
+    int rc = fqehip_convert_to_c128(precision, n, it.d_coeff0, 1.0, 0.0,
+                                    it.d_result, sv);
+    if (rc) return rc;
+    *nodes += 2;
+  }
+  const int nsec = static_cast<int>(grouped.size());
+  for (int step = any_apply ? 0 : 1; step <= nterms; ++step) {
+    int rc = launch_group_rows(d_secs, nsec, rows, max_lenb, step, st);
+    if (rc) return rc;
+    rc = launch_group_reduce(d_secs, nsec, blocks, step, 0.0,
+                             step ? -time / static_cast<double>(step) : 0.0, st);
+    if (rc) return rc;
+    *nodes += 2;
+  }
+  return FQEHIP_OK;
+}
+
 }  // namespace
 }  // namespace fqehip
 
@@ -85,7 +130,8 @@ using namespace fqehip;
 extern "C" int fqehip_batch_create(int precision, int nitems,
                                    const fqehip_batch_item* items,
                                    double time, int nterms, int nstreams,
-                                   void* stream, fqehip_batch** out) {
+                                   int group, void* stream,
+                                   fqehip_batch** out) {
   clear_error();
   FQEHIP_CHECK_ARG(out != nullptr, "NULL output handle");
   *out = nullptr;
@@ -132,13 +178,72 @@ extern "C" int fqehip_batch_create(int precision, int nitems,
     b->events.push_back(ev);
   }
   hipStream_t origin = b->streams[nb];
-  // longest processing time first, each to the least-loaded stream
-  std::vector<int> order(nitems);
-  std::iota(order.begin(), order.end(), 0);
+  // complex64 sectors of the rows4/rows8 kinds (fused, no split) run as
+  // one group: one row launch and one reduction per step for all of them
+  std::vector<int> grouped, single;
+  for (int i = 0; i < nitems; ++i) {
+    const fqehip_batch_item& it = items[i];
+    const bool g = group && precision == FQEHIP_C64 &&
+                   it.sector.lena * it.sector.lenb > 0 && !it.sector.hbb_hi &&
+                   it.sector.nslot == it.sector.nalpha * (it.sector.norb - it.sector.nalpha) &&
+                   fqehip_fused_supported(precision, &it.sector, &it.op);
+    (g ? grouped : single).push_back(i);
+  }
+  int64_t grows = 0, gblocks = 0, gmax_lenb = 0;
+  bool gapply = false;
+  double gwork = 0.0;
+  if (!grouped.empty()) {
+    std::vector<GroupSector> hs(grouped.size());
+    const int nt = group_reduce_threads();
+    for (size_t j = 0; j < grouped.size(); ++j) {
+      const fqehip_batch_item& it = items[grouped[j]];
+      const fqehip_sector& sc = it.sector;
+      const int64_t n = sc.lena * sc.lenb;
+      GroupSector& g = hs[j];
+      g.norb = sc.norb;
+      g.compressed = sc.npair == sc.norb * (sc.norb + 1) / 2 ? 1 : 0;
+      g.npair = sc.npair;
+      g.nslot = sc.nslot;
+      g.lena = sc.lena;
+      g.lenb = sc.lenb;
+      g.row0 = grows;
+      g.blk0 = gblocks;
+      g.amat = it.op.a;
+      g.pta = sc.ptab_a;
+      g.ptb = sc.ptab_b;
+      g.fwa = sc.fwd_a;
+      g.fwb = sc.fwd_b;
+      g.sla = sc.slot_a;
+      g.coeff0 = static_cast<const float*>(it.d_coeff0);
+      g.apply_out = static_cast<float*>(it.d_apply_out);
+      g.term = static_cast<float*>(it.d_term);
+      g.result = it.d_result;
+      g.slots = static_cast<float*>(it.d_workspace);
+      g.rowacc = g.slots + static_cast<int64_t>(sc.nslot) * 2 * n;
+      grows += sc.lena;
+      gblocks += (n + nt - 1) / nt;
+      gmax_lenb = std::max(gmax_lenb, sc.lenb);
+      gapply = gapply || it.d_apply_out != nullptr;
+      gwork += item_work(it);
+    }
+    const size_t bytes = hs.size() * sizeof(GroupSector);
+    if (hipMalloc(&b->d_group, bytes) != hipSuccess ||
+        hipMemcpy(b->d_group, hs.data(), bytes, hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      release(b);
+      set_error("batch: group descriptor upload failed");
+      return FQEHIP_ERR_HIP;
+    }
+    b->ngroup = static_cast<int>(grouped.size());
+  }
+  // longest processing time first, each to the least-loaded stream (the
+  // group occupies stream 0)
+  std::vector<int> order(single);
   std::sort(order.begin(), order.end(), [&](int x, int y) {
     return item_work(items[x]) > item_work(items[y]);
   });
   std::vector<double> load(nb, 0.0);
+  load[0] = gwork;
   std::vector<std::vector<int>> plan(nb);
   for (int i : order) {
     const int s = static_cast<int>(
@@ -157,6 +262,10 @@ extern "C" int fqehip_batch_create(int precision, int nitems,
   for (int s = 0; s < nb && rc == FQEHIP_OK; ++s)
     if (hipStreamWaitEvent(b->streams[s], b->events[nb], 0) != hipSuccess)
       rc = FQEHIP_ERR_HIP;
+  if (rc == FQEHIP_OK && !grouped.empty())
+    rc = enqueue_group(precision, items, grouped, b->d_group, grows, gblocks,
+                       gmax_lenb, gapply, time, nterms, b->streams[0],
+                       &b->nodes);
   for (int s = 0; s < nb && rc == FQEHIP_OK; ++s)
     for (int i : plan[s]) {
       rc = enqueue_item(precision, items[i], time, nterms, b->streams[s],
@@ -202,6 +311,10 @@ extern "C" int fqehip_batch_launch(fqehip_batch* b, void* stream) {
 
 extern "C" int fqehip_batch_nodes(const fqehip_batch* b) {
   return b ? b->nodes : 0;
+}
+
+extern "C" int fqehip_batch_grouped(const fqehip_batch* b) {
+  return b ? b->ngroup : 0;
 }
 
 extern "C" void fqehip_batch_destroy(fqehip_batch* b) { release(b); }

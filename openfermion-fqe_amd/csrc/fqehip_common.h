@@ -104,4 +104,27 @@ __device__ __forceinline__ int comp_pair(int i, int j) {
   return lo + hi * (hi + 1) / 2;
 }
 
+// One sector of a grouped batch launch (device array, built by the batch
+// code, read by k_fused_group / k_slot_reduce_group): the sector's fused
+// tables and operator, its buffers, and its first global row and first
+// reduction block (prefix sums over the group).
+struct GroupSector {
+  int norb, compressed, npair, nslot;
+  int64_t lena, lenb;
+  int64_t row0;                 // first global alpha row of this sector
+  int64_t blk0;                 // first block of the grouped reduction
+  const double* amat;
+  const int32_t* pta;
+  const int32_t* ptb;
+  const int32_t* fwa;
+  const int32_t* fwb;
+  const int32_t* sla;
+  const float* coeff0;          // step 0 input
+  float* apply_out;             // step 0 output; NULL: the sector has no step 0
+  float* term;                  // steps >= 1: input and output (in place)
+  double* result;               // steps >= 1: FP64 Taylor sum
+  float* slots;                 // workspace: slot rows
+  float* rowacc;                // workspace: row accumulators
+};
+
 }  // namespace fqehip

@@ -224,41 +224,17 @@ constexpr int kBfKSteps = kBfKPad / 32;
 constexpr int kBfStride = kBfKPad + 8;     // bf16 per LDS column (padding)
 constexpr int kBfCols = 2 * kBfTbC;        // re block then im block
 
-template <int NW, int WPE, bool kSplit>
-__global__ void __launch_bounds__(NW * 64, WPE)
-k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
-               const double* __restrict__ amat,
-               const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
-               const int32_t* __restrict__ fwa, const int32_t* __restrict__ fwb,
-               const int32_t* __restrict__ sla, int nslot,
-               const float* __restrict__ coeff, float* __restrict__ slots,
-               float* __restrict__ rowacc, const int32_t* __restrict__ done) {
-  constexpr int kT = NW * 64;              // threads
+// Per-row body and A' fragment setup of k_fused_bf16x3, shared with the
+// grouped launch over several sectors (k_fused_group).
+template <int NW, bool kSplit>
+__device__ __forceinline__ void fused_a_frags(
+    int npair, const double* __restrict__ amat,
+    bf16x8 (&ahi)[8 / NW][kBfKSteps], bf16x8 (&alo)[8 / NW][kBfKSteps]) {
   constexpr int kRT = 8 / NW;              // row tiles per wave
-  constexpr int kGroups = kT / kBfTbC;     // gather pair groups of 8
-  constexpr int kGIter = kBfKPad / (8 * kGroups);
-  extern __shared__ __align__(16) char smem[];
-  if (done && *done) return;
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6;
-  const int lane = tid & 63;
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int K = npair;
   const int M = npair + 1;
-  __bf16* dhi = reinterpret_cast<__bf16*>(smem);
-  __bf16* dlo = dhi + kBfCols * kBfStride;
-  RowTables rt;
-  rt.sbase = reinterpret_cast<int64_t*>(dlo + kBfCols * kBfStride);
-  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
-  rt.asgn = rt.aoff + kRowTabs;
-  rt.ssgn = rt.asgn + kRowTabs;
-  float* accrow = reinterpret_cast<float*>(rt.ssgn + kRowTabs);
-  double* drow = reinterpret_cast<double*>(rt.ssgn + kRowTabs);  // kSplit
-  const int Kg = kSplit ? K + 1 : K;       // gathered rows of the D tile
-  // k rows the MFMAs read (32 per step): small sectors skip the padding
-  const int ksteps = (Kg + 31) / 32;
-  const int kpad = 32 * ksteps;
-
-  bf16x8 ahi[kRT][kBfKSteps], alo[kRT][kBfKSteps];
 #pragma unroll
   for (int r = 0; r < kRT; ++r) {
     const int row = (wave * kRT + r) * 16 + (lane & 15);
@@ -280,6 +256,40 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
       }
     }
   }
+}
+
+template <int NW, bool kSplit>
+__device__ __forceinline__ void fused_row(
+    int64_t ia, int norb, int compressed, int npair, int64_t lena, int64_t lenb,
+    const double* __restrict__ amat, const int32_t* __restrict__ pta,
+    const int32_t* __restrict__ ptb, const int32_t* __restrict__ fwa,
+    const int32_t* __restrict__ fwb, const int32_t* __restrict__ sla, int nslot,
+    const float* __restrict__ coeff, float* __restrict__ slots,
+    float* __restrict__ rowacc, char* smem,
+    bf16x8 (&ahi)[8 / NW][kBfKSteps], bf16x8 (&alo)[8 / NW][kBfKSteps]) {
+  constexpr int kT = NW * 64;              // threads
+  constexpr int kRT = 8 / NW;              // row tiles per wave
+  constexpr int kGroups = kT / kBfTbC;     // gather pair groups of 8
+  constexpr int kGIter = kBfKPad / (8 * kGroups);
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int M = npair + 1;
+  __bf16* dhi = reinterpret_cast<__bf16*>(smem);
+  __bf16* dlo = dhi + kBfCols * kBfStride;
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(dlo + kBfCols * kBfStride);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  float* accrow = reinterpret_cast<float*>(rt.ssgn + kRowTabs);
+  double* drow = reinterpret_cast<double*>(rt.ssgn + kRowTabs);  // kSplit
+  const int Kg = kSplit ? K + 1 : K;       // gathered rows of the D tile
+  // k rows the MFMAs read (32 per step): small sectors skip the padding
+  const int ksteps = (Kg + 31) / 32;
+  const int kpad = 32 * ksteps;
+
   const bool wave_live = wave * kRT * 16 <= npair;
   const int t = tid % kBfTbC;            // gather column
   const int g = tid / kBfTbC;            // gather pair group
@@ -288,178 +298,196 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
   const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
   const __amdgpu_buffer_rsrc_t rfb = buffer_of(fwb, static_cast<int64_t>(K) * lenb * 4);
 
-  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
-    load_row_tables<float>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
-                           kT, kSplit ? K : -1);
-    if (kSplit) {
-      __syncthreads();
-      // row npair of A' for this alpha string: sum over occupied orbitals i
-      // (E^a_ii ia = ia, marked -2 in the row tables) of A'[ii][:]
-      for (int k = tid; k < Kg; k += kT) {
-        double v = 0.0;
-        for (int i = 0; i < norb; ++i) {
-          const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
-          if (rt.sbase[dp] == -2)
-            v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
-                       : amat[static_cast<int64_t>(npair) * K + dp];
-        }
-        drow[k] = v;
+
+  load_row_tables<float>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                         kT, kSplit ? K : -1);
+  if (kSplit) {
+    __syncthreads();
+    // row npair of A' for this alpha string: sum over occupied orbitals i
+    // (E^a_ii ia = ia, marked -2 in the row tables) of A'[ii][:]
+    for (int k = tid; k < Kg; k += kT) {
+      double v = 0.0;
+      for (int i = 0; i < norb; ++i) {
+        const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
+        if (rt.sbase[dp] == -2)
+          v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
+                     : amat[static_cast<int64_t>(npair) * K + dp];
       }
-      __syncthreads();
-#pragma unroll
-      for (int r = 0; r < kRT; ++r) {
-        if ((wave * kRT + r) * 16 + (lane & 15) == npair) {
-#pragma unroll
-          for (int kt = 0; kt < kBfKSteps; ++kt)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              const int kk = kt * 32 + 8 * (lane >> 4) + j;
-              const double v = kk < Kg ? drow[kk] : 0.0;
-              const __bf16 h = to_bf16(static_cast<float>(v));
-              ahi[r][kt][j] = h;
-              alo[r][kt][j] = to_bf16(static_cast<float>(v - static_cast<double>(from_bf16(h))));
-            }
-        }
-      }
-    } else {
-      for (int64_t x = tid; x < 2 * lenb; x += kT) accrow[x] = 0.0f;
+      drow[k] = v;
     }
-    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(float));
-    int32_t bt[kGIter][8];                 // gather-table entries, this chunk
-    auto load_bt = [&](int64_t ib0) {
-      const int ib = static_cast<int>(ib0) + t;
+    __syncthreads();
 #pragma unroll
-      for (int it = 0; it < kGIter; ++it)
+    for (int r = 0; r < kRT; ++r) {
+      if ((wave * kRT + r) * 16 + (lane & 15) == npair) {
+#pragma unroll
+        for (int kt = 0; kt < kBfKSteps; ++kt)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int kk = kt * 32 + 8 * (lane >> 4) + j;
+            const double v = kk < Kg ? drow[kk] : 0.0;
+            const __bf16 h = to_bf16(static_cast<float>(v));
+            ahi[r][kt][j] = h;
+            alo[r][kt][j] = to_bf16(static_cast<float>(v - static_cast<double>(from_bf16(h))));
+          }
+      }
+    }
+  } else {
+    for (int64_t x = tid; x < 2 * lenb; x += kT) accrow[x] = 0.0f;
+  }
+  const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(float));
+  int32_t bt[kGIter][8];                 // gather-table entries, this chunk
+  auto load_bt = [&](int64_t ib0) {
+    const int ib = static_cast<int>(ib0) + t;
+#pragma unroll
+    for (int it = 0; it < kGIter; ++it)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int q = 8 * (g + it * kGroups) + j;
+        bt[it][j] = q < kpad ? bload_i32(rpb, (ib < lenb && q < K)
+                                                  ? (q * static_cast<int>(lenb) + ib) * 4 : -1)
+                             : 0;
+      }
+  };
+  load_bt(0);
+  __syncthreads();
+  for (int chunk = 0; chunk < nchunks; ++chunk) {
+    const int64_t ib0 = static_cast<int64_t>(chunk) * kBfTbC;
+    {
+      const int ib = static_cast<int>(ib0) + t;
+      const bool colok = ib < lenb;
+#pragma unroll
+      for (int it = 0; it < kGIter; ++it) {
+        const int q0 = 8 * (g + it * kGroups);
+        if (q0 >= kpad) continue;        // rows no MFMA reads
+        bf16x8 hr, lr, hi, li;
+        float2 ca[8], cb[8];
+        float sa[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          const int q = 8 * (g + it * kGroups) + j;
-          bt[it][j] = q < kpad ? bload_i32(rpb, (ib < lenb && q < K)
-                                                    ? (q * static_cast<int>(lenb) + ib) * 4 : -1)
-                               : 0;
+          const int q = q0 + j;
+          const int32_t off = (colok && q < Kg) ? rt.aoff[q] : -1;
+          sa[j] = off >= 0 ? static_cast<float>(rt.asgn[q]) : 0.0f;
+          ca[j] = bload_f2(rc, off >= 0 ? (off + 2 * ib) * 4 : -1);
+          const int32_t b = bt[it][j];
+          cb[j] = bload_f2(rc, b ? crow_b + 8 * (iabs(b) - 1) : -1);
         }
-    };
-    load_bt(0);
-    __syncthreads();
-    for (int chunk = 0; chunk < nchunks; ++chunk) {
-      const int64_t ib0 = static_cast<int64_t>(chunk) * kBfTbC;
-      {
-        const int ib = static_cast<int>(ib0) + t;
-        const bool colok = ib < lenb;
 #pragma unroll
-        for (int it = 0; it < kGIter; ++it) {
-          const int q0 = 8 * (g + it * kGroups);
-          if (q0 >= kpad) continue;        // rows no MFMA reads
-          bf16x8 hr, lr, hi, li;
-          float2 ca[8], cb[8];
-          float sa[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int q = q0 + j;
-            const int32_t off = (colok && q < Kg) ? rt.aoff[q] : -1;
-            sa[j] = off >= 0 ? static_cast<float>(rt.asgn[q]) : 0.0f;
-            ca[j] = bload_f2(rc, off >= 0 ? (off + 2 * ib) * 4 : -1);
-            const int32_t b = bt[it][j];
-            cb[j] = bload_f2(rc, b ? crow_b + 8 * (iabs(b) - 1) : -1);
-          }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const float sb = (kSplit ? 2.0f : 1.0f) * sgnf(bt[it][j]);
-            const float re = sa[j] * ca[j].x + sb * cb[j].x;
-            const float im = sa[j] * ca[j].y + sb * cb[j].y;
-            const __bf16 h0 = to_bf16(re), h1 = to_bf16(im);
-            hr[j] = h0;
-            lr[j] = to_bf16(re - from_bf16(h0));
-            hi[j] = h1;
-            li[j] = to_bf16(im - from_bf16(h1));
-          }
-          *reinterpret_cast<bf16x8*>(dhi + t * kBfStride + q0) = hr;
-          *reinterpret_cast<bf16x8*>(dlo + t * kBfStride + q0) = lr;
-          *reinterpret_cast<bf16x8*>(dhi + (kBfTbC + t) * kBfStride + q0) = hi;
-          *reinterpret_cast<bf16x8*>(dlo + (kBfTbC + t) * kBfStride + q0) = li;
+        for (int j = 0; j < 8; ++j) {
+          const float sb = (kSplit ? 2.0f : 1.0f) * sgnf(bt[it][j]);
+          const float re = sa[j] * ca[j].x + sb * cb[j].x;
+          const float im = sa[j] * ca[j].y + sb * cb[j].y;
+          const __bf16 h0 = to_bf16(re), h1 = to_bf16(im);
+          hr[j] = h0;
+          lr[j] = to_bf16(re - from_bf16(h0));
+          hi[j] = h1;
+          li[j] = to_bf16(im - from_bf16(h1));
         }
+        *reinterpret_cast<bf16x8*>(dhi + t * kBfStride + q0) = hr;
+        *reinterpret_cast<bf16x8*>(dlo + t * kBfStride + q0) = lr;
+        *reinterpret_cast<bf16x8*>(dhi + (kBfTbC + t) * kBfStride + q0) = hi;
+        *reinterpret_cast<bf16x8*>(dlo + (kBfTbC + t) * kBfStride + q0) = li;
       }
-      __syncthreads();
-      // the next chunk's gather table is in flight during the MFMAs
-      if (chunk + 1 < nchunks) load_bt(ib0 + kBfTbC);
-      if (wave_live) {
-        // two passes over 16-column halves h: tile h holds Re, tile h+2 Im
+    }
+    __syncthreads();
+    // the next chunk's gather table is in flight during the MFMAs
+    if (chunk + 1 < nchunks) load_bt(ib0 + kBfTbC);
+    if (wave_live) {
+      // two passes over 16-column halves h: tile h holds Re, tile h+2 Im
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          int32_t fbv[kRT][4];
+      for (int h = 0; h < 2; ++h) {
+        int32_t fbv[kRT][4];
 #pragma unroll
-          for (int r = 0; r < kRT; ++r)
+        for (int r = 0; r < kRT; ++r)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
-              const int jb = static_cast<int>(ib0) + h * 16 + (lane & 15);
-              fbv[r][i] = kSplit ? 0 : bload_i32(rfb, (p < npair && jb < lenb)
-                                             ? (p * static_cast<int>(lenb) + jb) * 4 : -1);
-            }
-          floatx4 are[kRT], aim[kRT];
+          for (int i = 0; i < 4; ++i) {
+            const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
+            const int jb = static_cast<int>(ib0) + h * 16 + (lane & 15);
+            fbv[r][i] = kSplit ? 0 : bload_i32(rfb, (p < npair && jb < lenb)
+                                           ? (p * static_cast<int>(lenb) + jb) * 4 : -1);
+          }
+        floatx4 are[kRT], aim[kRT];
+#pragma unroll
+        for (int r = 0; r < kRT; ++r) {
+          are[r] = floatx4{0, 0, 0, 0};
+          aim[r] = floatx4{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int kt = 0; kt < kBfKSteps; ++kt) {
+          if (kt >= ksteps) break;
+          const int kq = kt * 32 + 8 * (lane >> 4);
+          const int cre = h * 16 + (lane & 15);
+          const int cim = kBfTbC + cre;
+          const bf16x8 bhr = *reinterpret_cast<const bf16x8*>(dhi + cre * kBfStride + kq);
+          const bf16x8 blr = *reinterpret_cast<const bf16x8*>(dlo + cre * kBfStride + kq);
+          const bf16x8 bhi = *reinterpret_cast<const bf16x8*>(dhi + cim * kBfStride + kq);
+          const bf16x8 bli = *reinterpret_cast<const bf16x8*>(dlo + cim * kBfStride + kq);
 #pragma unroll
           for (int r = 0; r < kRT; ++r) {
-            are[r] = floatx4{0, 0, 0, 0};
-            aim[r] = floatx4{0, 0, 0, 0};
+            are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhr, are[r], 0, 0, 0);
+            aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhi, aim[r], 0, 0, 0);
+            are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], blr, are[r], 0, 0, 0);
+            aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bli, aim[r], 0, 0, 0);
+            are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhr, are[r], 0, 0, 0);
+            aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhi, aim[r], 0, 0, 0);
           }
-#pragma unroll
-          for (int kt = 0; kt < kBfKSteps; ++kt) {
-            if (kt >= ksteps) break;
-            const int kq = kt * 32 + 8 * (lane >> 4);
-            const int cre = h * 16 + (lane & 15);
-            const int cim = kBfTbC + cre;
-            const bf16x8 bhr = *reinterpret_cast<const bf16x8*>(dhi + cre * kBfStride + kq);
-            const bf16x8 blr = *reinterpret_cast<const bf16x8*>(dlo + cre * kBfStride + kq);
-            const bf16x8 bhi = *reinterpret_cast<const bf16x8*>(dhi + cim * kBfStride + kq);
-            const bf16x8 bli = *reinterpret_cast<const bf16x8*>(dlo + cim * kBfStride + kq);
-#pragma unroll
-            for (int r = 0; r < kRT; ++r) {
-              are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhr, are[r], 0, 0, 0);
-              aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bhi, aim[r], 0, 0, 0);
-              are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], blr, are[r], 0, 0, 0);
-              aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[r][kt], bli, aim[r], 0, 0, 0);
-              are[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhr, are[r], 0, 0, 0);
-              aim[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo[r][kt], bhi, aim[r], 0, 0, 0);
-            }
-          }
-          // f32 MFMA layout: item i of lane l is row 4*(l/16)+i, col l%16
-#pragma unroll
-          for (int r = 0; r < kRT; ++r)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
-              const int64_t jb = ib0 + h * 16 + (lane & 15);
-              if (kSplit) {
-                if (jb >= lenb || p > npair) continue;
-                if (p == npair) {
-                  float2* d = reinterpret_cast<float2*>(rowacc + (ia * lenb + jb) * 2);
-                  float2 o = *d;
-                  o.x += are[r][i];
-                  o.y += aim[r][i];
-                  *d = o;
-                } else {
-                  const int64_t sb = rt.sbase[p];
-                  if (sb >= 0) {
-                    const float sg = static_cast<float>(rt.ssgn[p]);
-                    float* dst = slots + sb + 2 * jb;
-                    dst[0] = sg * are[r][i];
-                    dst[1] = sg * aim[r][i];
-                  }
-                }
-              } else if (p <= npair && jb < lenb) {
-                scatter_complex<float, float>(are[r][i], aim[r][i], p, npair,
-                                              jb, fbv[r][i], rt, accrow, slots);
-              }
-            }
         }
+        // f32 MFMA layout: item i of lane l is row 4*(l/16)+i, col l%16
+#pragma unroll
+        for (int r = 0; r < kRT; ++r)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int p = (wave * kRT + r) * 16 + 4 * (lane >> 4) + i;
+            const int64_t jb = ib0 + h * 16 + (lane & 15);
+            if (kSplit) {
+              if (jb >= lenb || p > npair) continue;
+              if (p == npair) {
+                float2* d = reinterpret_cast<float2*>(rowacc + (ia * lenb + jb) * 2);
+                float2 o = *d;
+                o.x += are[r][i];
+                o.y += aim[r][i];
+                *d = o;
+              } else {
+                const int64_t sb = rt.sbase[p];
+                if (sb >= 0) {
+                  const float sg = static_cast<float>(rt.ssgn[p]);
+                  float* dst = slots + sb + 2 * jb;
+                  dst[0] = sg * are[r][i];
+                  dst[1] = sg * aim[r][i];
+                }
+              }
+            } else if (p <= npair && jb < lenb) {
+              scatter_complex<float, float>(are[r][i], aim[r][i], p, npair,
+                                            jb, fbv[r][i], rt, accrow, slots);
+            }
+          }
       }
-      __syncthreads();
-    }
-    if (!kSplit) {
-      float* dst = rowacc + ia * 2 * lenb;
-      for (int64_t x = tid; x < 2 * lenb; x += kT) dst[x] = accrow[x];
     }
     __syncthreads();
   }
+  if (!kSplit) {
+    float* dst = rowacc + ia * 2 * lenb;
+    for (int64_t x = tid; x < 2 * lenb; x += kT) dst[x] = accrow[x];
+  }
+  __syncthreads();
+}
+
+template <int NW, int WPE, bool kSplit>
+__global__ void __launch_bounds__(NW * 64, WPE)
+k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
+               const double* __restrict__ amat,
+               const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+               const int32_t* __restrict__ fwa, const int32_t* __restrict__ fwb,
+               const int32_t* __restrict__ sla, int nslot,
+               const float* __restrict__ coeff, float* __restrict__ slots,
+               float* __restrict__ rowacc, const int32_t* __restrict__ done) {
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  bf16x8 ahi[8 / NW][kBfKSteps], alo[8 / NW][kBfKSteps];
+  fused_a_frags<NW, kSplit>(npair, amat, ahi, alo);
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x)
+    fused_row<NW, kSplit>(ia, norb, compressed, npair, lena, lenb, amat, pta,
+                          ptb, fwa, fwb, sla, nslot, coeff, slots, rowacc, smem,
+                          ahi, alo);
 }
 
 // ---------------- split kernel over the live pair rows ------------------------
@@ -1221,6 +1249,60 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
 // ---------------------------------------------------------------------------
 // slot reduction + epilogue
 // ---------------------------------------------------------------------------
+// Element idx = (ia, ib) of a slot reduction: rowacc + the nslot slots in
+// FP64, out = alpha * that + beta * cin, acc += the FP64 value; returns
+// |out|^2 (shared by k_slot_reduce and k_slot_reduce_group).
+template <typename T>
+__device__ __forceinline__ double slot_element(
+    int64_t idx, int64_t ia, int64_t ib, int nslot, int64_t lenb,
+    const T* __restrict__ slots, const T* __restrict__ rowacc,
+    double alpha_re, double alpha_im, double beta_re, double beta_im,
+    const T* cin, T* out, double* __restrict__ acc) {
+  double re = static_cast<double>(rowacc[2 * idx]);
+  double im = static_cast<double>(rowacc[2 * idx + 1]);
+  const T* s = slots + (ia * nslot * lenb + ib) * 2;
+  // complex64 slots: eight loads in flight per thread (complex128 slots
+  // keep one, which measured faster); the FP64 sum keeps slot order
+  constexpr int kU = sizeof(T) == 4 ? 8 : 1;
+  int k = 0;
+  for (; k + kU <= nslot; k += kU) {
+    T vr[kU], vi[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      vr[u] = s[2 * lenb * u];
+      vi[u] = s[2 * lenb * u + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      re += static_cast<double>(vr[u]);
+      im += static_cast<double>(vi[u]);
+    }
+    s += 2 * lenb * kU;
+  }
+  for (; k < nslot; ++k) {
+    re += static_cast<double>(s[0]);
+    im += static_cast<double>(s[1]);
+    s += 2 * lenb;
+  }
+  double vr = alpha_re * re - alpha_im * im;
+  double vi = alpha_re * im + alpha_im * re;
+  if (beta_re != 0.0 || beta_im != 0.0) {
+    const double cr = static_cast<double>(cin[2 * idx]);
+    const double ci = static_cast<double>(cin[2 * idx + 1]);
+    vr += beta_re * cr - beta_im * ci;
+    vi += beta_re * ci + beta_im * cr;
+  }
+  const T orr = static_cast<T>(vr);
+  const T oi = static_cast<T>(vi);
+  out[2 * idx] = orr;
+  out[2 * idx + 1] = oi;
+  if (acc) {
+    acc[2 * idx] += vr;
+    acc[2 * idx + 1] += vi;
+  }
+  return static_cast<double>(orr) * orr + static_cast<double>(oi) * oi;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
@@ -1237,49 +1319,8 @@ k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
     int64_t ia = static_cast<int64_t>(static_cast<double>(idx) * inv_lenb);
     int64_t ib = idx - ia * lenb;
     if (ib < 0) { --ia; ib += lenb; } else if (ib >= lenb) { ++ia; ib -= lenb; }
-    double re = static_cast<double>(rowacc[2 * idx]);
-    double im = static_cast<double>(rowacc[2 * idx + 1]);
-    const T* s = slots + (ia * nslot * lenb + ib) * 2;
-    // complex64 slots: eight loads in flight per thread (complex128 slots
-    // keep one, which measured faster); the FP64 sum keeps slot order
-    constexpr int kU = sizeof(T) == 4 ? 8 : 1;
-    int k = 0;
-    for (; k + kU <= nslot; k += kU) {
-      T vr[kU], vi[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        vr[u] = s[2 * lenb * u];
-        vi[u] = s[2 * lenb * u + 1];
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        re += static_cast<double>(vr[u]);
-        im += static_cast<double>(vi[u]);
-      }
-      s += 2 * lenb * kU;
-    }
-    for (; k < nslot; ++k) {
-      re += static_cast<double>(s[0]);
-      im += static_cast<double>(s[1]);
-      s += 2 * lenb;
-    }
-    double vr = alpha_re * re - alpha_im * im;
-    double vi = alpha_re * im + alpha_im * re;
-    if (beta_re != 0.0 || beta_im != 0.0) {
-      const double cr = static_cast<double>(cin[2 * idx]);
-      const double ci = static_cast<double>(cin[2 * idx + 1]);
-      vr += beta_re * cr - beta_im * ci;
-      vi += beta_re * ci + beta_im * cr;
-    }
-    const T orr = static_cast<T>(vr);
-    const T oi = static_cast<T>(vi);
-    out[2 * idx] = orr;
-    out[2 * idx + 1] = oi;
-    if (acc) {
-      acc[2 * idx] += vr;
-      acc[2 * idx + 1] += vi;
-    }
-    sq = static_cast<double>(orr) * orr + static_cast<double>(oi) * oi;
+    sq = slot_element<T>(idx, ia, ib, nslot, lenb, slots, rowacc, alpha_re,
+                         alpha_im, beta_re, beta_im, cin, out, acc);
   }
   if (partials) {
     for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
@@ -1291,6 +1332,75 @@ k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
       partials[blockIdx.x] = t;
     }
   }
+}
+
+// ---------------- grouped launches over the sectors of a batch --------------
+// One launch covers one Taylor step of every sector of the group: global
+// alpha row gr belongs to the last sector with row0 <= gr (a uniform
+// binary search per row), which then runs the rows4/rows8 row body.  The
+// rows of all sectors together fill the GPU where each small sector alone
+// would not.
+template <int NW, int WPE>
+__global__ void __launch_bounds__(NW * 64, WPE)
+k_fused_group(const GroupSector* __restrict__ gs, int nsec, int64_t rows,
+              int step) {
+  extern __shared__ __align__(16) char smem[];
+  bf16x8 ahi[8 / NW][kBfKSteps], alo[8 / NW][kBfKSteps];
+  // a contiguous run of rows per workgroup: the A' fragments are rebuilt
+  // only where the run crosses into another sector
+  const int64_t per = (rows + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * per;
+  const int64_t r1 = r0 + per < rows ? r0 + per : rows;
+  int lo = 0;
+  if (r0 < rows) {
+    int hi = nsec - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (gs[mid].row0 <= r0) lo = mid;
+      else hi = mid - 1;
+    }
+  }
+  int cur = -1;
+  for (int64_t gr = r0; gr < r1; ++gr) {
+    while (lo + 1 < nsec && gs[lo + 1].row0 <= gr) ++lo;
+    const GroupSector& sc = gs[lo];
+    if (step == 0 && sc.apply_out == nullptr) continue;
+    if (lo != cur) {
+      fused_a_frags<NW, false>(sc.npair, sc.amat, ahi, alo);
+      cur = lo;
+    }
+    fused_row<NW, false>(gr - sc.row0, sc.norb, sc.compressed, sc.npair,
+                         sc.lena, sc.lenb, sc.amat, sc.pta, sc.ptb, sc.fwa,
+                         sc.fwb, sc.sla, sc.nslot,
+                         step == 0 ? sc.coeff0 : sc.term, sc.slots, sc.rowacc,
+                         smem, ahi, alo);
+  }
+}
+
+// Step 0: apply_out = rowacc + slots.  Step k >= 1: term = (ar + i ai)
+// (rowacc + slots) in place and result += term in FP64.  Block gb belongs
+// to the last sector with blk0 <= gb.
+__global__ void __launch_bounds__(kThreads)
+k_slot_reduce_group(const GroupSector* __restrict__ gs, int nsec, int step,
+                    double ar, double ai) {
+  const int64_t gb = blockIdx.x;
+  int lo = 0, hi = nsec - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gs[mid].blk0 <= gb) lo = mid;
+    else hi = mid - 1;
+  }
+  const GroupSector& sc = gs[lo];
+  float* out = step == 0 ? sc.apply_out : sc.term;
+  if (out == nullptr) return;
+  const int64_t idx = (gb - sc.blk0) * kThreads + threadIdx.x;
+  if (idx >= sc.lena * sc.lenb) return;
+  const int64_t ia = idx / sc.lenb;
+  const int64_t ib = idx - ia * sc.lenb;
+  (void)slot_element<float>(idx, ia, ib, sc.nslot, sc.lenb, sc.slots,
+                            sc.rowacc, step == 0 ? 1.0 : ar,
+                            step == 0 ? 0.0 : ai, 0.0, 0.0, nullptr, out,
+                            step == 0 ? nullptr : sc.result);
 }
 
 size_t row_tables_bytes() {
@@ -1478,6 +1588,40 @@ int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
   FQEHIP_CHECK_LAUNCH("k_slot_reduce");
   return FQEHIP_OK;
 }
+
+// Grouped launches (batch code): rows of every sector of the group, then
+// their reductions; rows8 row body, LDS sized for the longest beta row.
+int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
+                      int64_t max_lenb, int step, hipStream_t st) {
+  if (rows == 0 || nsec == 0) return FQEHIP_OK;
+  const size_t lds = bf16_lds_bytes(max_lenb);
+  if (lds > kLdsLimit) {
+    set_error("grouped rows: lenb=%lld exceeds the LDS row accumulator",
+              static_cast<long long>(max_lenb));
+    return FQEHIP_ERR_INVALID;
+  }
+  // about four workgroups per CU at a time, several rounds: runs of
+  // consecutive rows while the load stays balanced
+  const unsigned grid = static_cast<unsigned>(rows < 4096 ? rows : 4096);
+  k_fused_group<8, 2><<<grid, 512, lds, st>>>(d_secs, nsec, rows, step);
+  FQEHIP_CHECK_LAUNCH("k_fused_group");
+  return FQEHIP_OK;
+}
+
+int launch_group_reduce(const GroupSector* d_secs, int nsec, int64_t blocks,
+                        int step, double ar, double ai, hipStream_t st) {
+  if (blocks == 0 || nsec == 0) return FQEHIP_OK;
+  if (blocks > 0x7fffffffLL) {
+    set_error("grouped reduction: %lld blocks", static_cast<long long>(blocks));
+    return FQEHIP_ERR_INVALID;
+  }
+  k_slot_reduce_group<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(
+      d_secs, nsec, step, ar, ai);
+  FQEHIP_CHECK_LAUNCH("k_slot_reduce_group");
+  return FQEHIP_OK;
+}
+
+int group_reduce_threads() { return kThreads; }
 
 }  // namespace fqehip
 

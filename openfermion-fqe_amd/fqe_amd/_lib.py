@@ -96,10 +96,11 @@ _SIGNATURES = {
     "fqehip_apply_workspace_bytes": (c_int64, [c_int, POINTER(Sector)]),
     "fqehip_beta_block_ld": (c_int64, [c_int64]),
     "fqehip_batch_create": (c_int, [c_int, c_int, POINTER(BatchItem), c_double,
-                                    c_int, c_int, c_void_p,
+                                    c_int, c_int, c_int, c_void_p,
                                     POINTER(c_void_p)]),
     "fqehip_batch_launch": (c_int, [c_void_p, c_void_p]),
     "fqehip_batch_nodes": (c_int, [c_void_p]),
+    "fqehip_batch_grouped": (c_int, [c_void_p]),
     "fqehip_batch_destroy": (None, [c_void_p]),
     "fqehip_diagonal": (c_int, [c_int, c_int64, c_int64, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_void_p, c_void_p,
@@ -188,7 +189,7 @@ def ptr(t) -> int:
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
     if name.endswith(("_count", "_bytes", "_supported", "_ld", "_floats",
-                      "_nodes", "_destroy")):
+                      "_nodes", "_grouped", "_destroy")):
         return rc
     if rc != OK:
         msg = lib().fqehip_last_error().decode(errors="replace")

@@ -144,12 +144,14 @@ def autotune(specs: Sequence[SectorSpec], dtype: torch.dtype = torch.complex64,
 
 class BatchedTaylor:
     """One apply + ``nterms``-term Taylor series of exp(-i t H) for every
-    sector, recorded into one HIP graph."""
+    sector, recorded into one HIP graph.  With ``group`` (complex64), the
+    sectors of the rows4/rows8 variants run as one group: one row launch
+    and one reduction per step for all of them."""
 
     def __init__(self, specs: Sequence[SectorSpec], time_step: float,
                  nterms: int = 10, dtype: torch.dtype = torch.complex64,
                  tiles: Optional[Dict[str, str]] = None,
-                 nstreams: int = 8) -> None:
+                 nstreams: int = 8, group: bool = True) -> None:
         self.time = float(time_step)
         self.nterms = int(nterms)
         self.dtype = dtype
@@ -160,6 +162,7 @@ class BatchedTaylor:
             for sp in specs]
         self.prec = engine.precision_of(dtype)
         self.nstreams = nstreams
+        self.group = bool(group)
         self._batch = None
 
     def build(self) -> float:
@@ -170,13 +173,17 @@ class BatchedTaylor:
         handle = ctypes.c_void_p()
         torch.cuda.synchronize()
         _lib.call("fqehip_batch_create", self.prec, len(self.sectors), items,
-                  self.time, self.nterms, self.nstreams, _lib.stream_ptr(),
-                  ctypes.byref(handle))
+                  self.time, self.nterms, self.nstreams, int(self.group),
+                  _lib.stream_ptr(), ctypes.byref(handle))
         self._batch = handle
         return time.perf_counter() - t0
 
     def nodes(self) -> int:
         return int(_lib.call("fqehip_batch_nodes", self._batch))
+
+    def grouped(self) -> int:
+        """Number of sectors that run as one group."""
+        return int(_lib.call("fqehip_batch_grouped", self._batch))
 
     def launch(self) -> None:
         _lib.call("fqehip_batch_launch", self._batch, _lib.stream_ptr())

@@ -72,6 +72,53 @@ def test_batched_graph_matches_oracle_and_sequential(fqe, dtype):
     bt.close()
 
 
+def test_grouped_batch_matches_oracle_and_sequential(fqe):
+    """complex64 sectors of the rows4/rows8 variants run as one group (one
+    row launch and one reduction per Taylor step for all of them) beside
+    split and unfused sectors on their own kernels: every sector against
+    the oracle, replays against each other, and the group against the
+    per-sector kernels of run_sequential."""
+    import torch
+    from fqe_amd import batched
+    specs = batched.random_specs(24, seed=44, norb_range=(4, 9))
+    cycle = ["rows8", "rows4", "rows8", "split", "unfused"]
+    tiles, want = {}, 0
+    for i, sp in enumerate(specs):
+        key = f"{sp.norb},{sp.nalpha},{sp.nbeta}"
+        if key not in tiles:
+            tiles[key] = cycle[i % len(cycle)]
+    for sp in specs:
+        if tiles[f"{sp.norb},{sp.nalpha},{sp.nbeta}"] in ("rows4", "rows8"):
+            want += 1
+    print("variants", tiles, flush=True)
+    bt = batched.BatchedTaylor(specs, 0.1, nterms=10, dtype=torch.complex64,
+                               tiles=tiles, nstreams=4, group=True)
+    bt.build()
+    assert bt.grouped() == want > 0
+    refs = [_oracle(sp, 0.1, 10) for sp in specs]
+    for rep in range(2):
+        bt.launch()
+        got = bt.results()
+        for (ap, ev), (rap, rev), sp in zip(got, refs, specs):
+            assert rel(ap, rap) < MIXED_TOL, (rep, sp.norb, sp.nalpha, sp.nbeta)
+            assert rel(ev, rev) < MIXED_TOL, (rep, sp.norb, sp.nalpha, sp.nbeta)
+        # the rows4/rows8 row body accumulates beta excitations with LDS
+        # atomics: replays round differently, and 10 Taylor terms carry it
+        if rep == 0:
+            first = got
+        else:
+            for (a0, e0), (a1, e1) in zip(first, got):
+                assert rel(a1, a0) < 1e-5 and rel(e1, e0) < 1e-5
+        torch.zeros(1000, device="cuda").sum().item()
+    # run_sequential takes each sector's own variant (rows4 sectors on the
+    # 4-wave kernel, the group always on the rows8 body): the beta scatter
+    # rounds in another order, so the Taylor sums agree to mixed rounding
+    bt.run_sequential()
+    for (a0, e0), (a1, e1) in zip(first, bt.results()):
+        assert rel(a1, a0) < 2e-5 and rel(e1, e0) < 2e-5
+    bt.close()
+
+
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_every_variant_matches_oracle(fqe, dtype):
     """Each kernel variant of each precision (complex128: rows4, split,
