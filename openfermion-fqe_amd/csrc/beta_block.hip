@@ -303,9 +303,11 @@ k_beta_gemm_f64(int64_t lena, int64_t lenb, const double* __restrict__ hcol,
                 const double* __restrict__ coeff, double* __restrict__ out,
                 const int32_t* __restrict__ done, int ta, int tb,
                 int per_xcd) {
-  __shared__ __align__(16) double cre[kFK * kFS];
-  __shared__ __align__(16) double cim[kFK * kFS];
-  __shared__ __align__(16) double hb[kFK * kFS];
+  // two buffers of each tile: the next K step is stored while the current
+  // one feeds the MFMAs (one barrier per step)
+  __shared__ __align__(16) double cre[2][kFK * kFS];
+  __shared__ __align__(16) double cim[2][kFK * kFS];
+  __shared__ __align__(16) double hb[2][kFK * kFS];
   if (done && *done) return;
   int tile_a, tile_b;
   if (!xcd_tile(ta, tb, per_xcd, &tile_a, &tile_b)) return;
@@ -347,17 +349,17 @@ k_beta_gemm_f64(int64_t lena, int64_t lenb, const double* __restrict__ hcol,
       hv[j] = static_cast<uint64_t>(y[0]) | (static_cast<uint64_t>(y[1]) << 32);
     }
   };
-  auto store = [&]() {
+  auto store = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint64_t re = static_cast<uint64_t>(cv[j][0]) | (static_cast<uint64_t>(cv[j][1]) << 32);
       const uint64_t im = static_cast<uint64_t>(cv[j][2]) | (static_cast<uint64_t>(cv[j][3]) << 32);
-      cre[(ck + j) * kFS + cr] = __longlong_as_double(static_cast<long long>(re));
-      cim[(ck + j) * kFS + cr] = __longlong_as_double(static_cast<long long>(im));
+      cre[buf][(ck + j) * kFS + cr] = __longlong_as_double(static_cast<long long>(re));
+      cim[buf][(ck + j) * kFS + cr] = __longlong_as_double(static_cast<long long>(im));
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      hb[hk * kFS + hn + j] = __longlong_as_double(static_cast<long long>(hv[j]));
+      hb[buf][hk * kFS + hn + j] = __longlong_as_double(static_cast<long long>(hv[j]));
   };
 
   double4_t accr[2][4], acci[2][4];
@@ -371,11 +373,11 @@ k_beta_gemm_f64(int64_t lena, int64_t lenb, const double* __restrict__ hcol,
 
   const int64_t nk = (lenb + kFK - 1) / kFK;
   load(0);
+  store(0);
+  if (nk > 1) load(kFK);                  // step 1 in flight
+  __syncthreads();
   for (int64_t ks = 0; ks < nk; ++ks) {
-    __syncthreads();                      // previous step's LDS reads done
-    store();
-    __syncthreads();
-    if (ks + 1 < nk) load((ks + 1) * kFK);  // next tiles in flight
+    const int cb = static_cast<int>(ks & 1);
     // f64 MFMA operands: A[r = l%16][k = l/16], B[k = l/16][c = l%16]
 #pragma unroll
     for (int kq = 0; kq < kFK / 4; ++kq) {
@@ -383,11 +385,11 @@ k_beta_gemm_f64(int64_t lena, int64_t lenb, const double* __restrict__ hcol,
       double ar[2], ai[2], bv[4];
 #pragma unroll
       for (int r = 0; r < 2; ++r) {
-        ar[r] = cre[krow + wa + 16 * r + (lane & 15)];
-        ai[r] = cim[krow + wa + 16 * r + (lane & 15)];
+        ar[r] = cre[cb][krow + wa + 16 * r + (lane & 15)];
+        ai[r] = cim[cb][krow + wa + 16 * r + (lane & 15)];
       }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) bv[c] = hb[krow + wb + 16 * c + (lane & 15)];
+      for (int c = 0; c < 4; ++c) bv[c] = hb[cb][krow + wb + 16 * c + (lane & 15)];
 #pragma unroll
       for (int r = 0; r < 2; ++r)
 #pragma unroll
@@ -396,6 +398,13 @@ k_beta_gemm_f64(int64_t lena, int64_t lenb, const double* __restrict__ hcol,
           acci[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[r], bv[c], acci[r][c], 0, 0, 0);
         }
     }
+    // step ks+1 into the other buffer (its readers finished at the last
+    // barrier), step ks+2 in flight, then one barrier
+    if (ks + 1 < nk) {
+      store(cb ^ 1);
+      if (ks + 2 < nk) load((ks + 2) * kFK);
+    }
+    __syncthreads();
   }
   // f64 MFMA layout: item i of lane l is row l/16 + 4 i, column l%16
 #pragma unroll
