@@ -153,14 +153,16 @@ __device__ __forceinline__ bool xcd_tile(int ta, int tb, int per_xcd,
   return true;
 }
 
-__global__ void __launch_bounds__(256, 1)
+__global__ void __launch_bounds__(256, 2)
 k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
                    const __bf16* __restrict__ hhi,
                    const __bf16* __restrict__ hlo,
                    const float* __restrict__ coeff, float* __restrict__ out,
                    const int32_t* __restrict__ done, int ta, int tb,
                    int per_xcd) {
-  __shared__ __align__(16) __bf16 sa[4][kGT * kGStride];  // hr, lr, hi, li
+  // two buffers of the split C tile (hr, lr, hi, li): the next K step is
+  // stored while the current one feeds the MFMAs (one barrier per step)
+  __shared__ __align__(16) __bf16 sa[2][4][kGT * kGStride];
   if (done && *done) return;
   int tile_a, tile_b;
   if (!xcd_tile(ta, tb, per_xcd, &tile_a, &tile_b)) return;
@@ -201,7 +203,7 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
       cv[4 * v + 3] = __uint_as_float(x[3]);
     }
   };
-  auto store_c = [&]() {
+  auto store_c = [&](int buf) {
     bf16x8 hr0, lr0, hi0, li0, hr1, lr1, hi1, li1;
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
@@ -212,14 +214,14 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
       else { hr1[j - 8] = h; lr1[j - 8] = l; hi1[j - 8] = g; li1[j - 8] = m; }
     }
     const int o = lr * kGStride + lk;
-    *reinterpret_cast<bf16x8*>(&sa[0][o]) = hr0;
-    *reinterpret_cast<bf16x8*>(&sa[0][o + 8]) = hr1;
-    *reinterpret_cast<bf16x8*>(&sa[1][o]) = lr0;
-    *reinterpret_cast<bf16x8*>(&sa[1][o + 8]) = lr1;
-    *reinterpret_cast<bf16x8*>(&sa[2][o]) = hi0;
-    *reinterpret_cast<bf16x8*>(&sa[2][o + 8]) = hi1;
-    *reinterpret_cast<bf16x8*>(&sa[3][o]) = li0;
-    *reinterpret_cast<bf16x8*>(&sa[3][o + 8]) = li1;
+    *reinterpret_cast<bf16x8*>(&sa[buf][0][o]) = hr0;
+    *reinterpret_cast<bf16x8*>(&sa[buf][0][o + 8]) = hr1;
+    *reinterpret_cast<bf16x8*>(&sa[buf][1][o]) = lr0;
+    *reinterpret_cast<bf16x8*>(&sa[buf][1][o + 8]) = lr1;
+    *reinterpret_cast<bf16x8*>(&sa[buf][2][o]) = hi0;
+    *reinterpret_cast<bf16x8*>(&sa[buf][2][o + 8]) = hi1;
+    *reinterpret_cast<bf16x8*>(&sa[buf][3][o]) = li0;
+    *reinterpret_cast<bf16x8*>(&sa[buf][3][o + 8]) = li1;
   };
   // Hbb operand registers for one K step: 4 column tiles x (hi, lo)
   bf16x8 bh[4], bl[4];
@@ -246,20 +248,20 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
 
   const int64_t nk = (lenb + kGK - 1) / kGK;
   load_c(0);
+  store_c(0);
+  if (nk > 1) load_c(kGK);                // step 1 in flight
+  __syncthreads();
   for (int64_t ks = 0; ks < nk; ++ks) {
     const int64_t k0 = ks * kGK;
-    __syncthreads();                      // previous step's LDS reads done
-    store_c();
+    const int cb = static_cast<int>(ks & 1);
     load_h(k0);
-    __syncthreads();
-    if (ks + 1 < nk) load_c(k0 + kGK);    // next C tile in flight
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int o = (wa + 16 * r + (lane & 15)) * kGStride + 8 * (lane >> 4);
-      const bf16x8 ahr = *reinterpret_cast<const bf16x8*>(&sa[0][o]);
-      const bf16x8 alr = *reinterpret_cast<const bf16x8*>(&sa[1][o]);
-      const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&sa[2][o]);
-      const bf16x8 ali = *reinterpret_cast<const bf16x8*>(&sa[3][o]);
+      const bf16x8 ahr = *reinterpret_cast<const bf16x8*>(&sa[cb][0][o]);
+      const bf16x8 alr = *reinterpret_cast<const bf16x8*>(&sa[cb][1][o]);
+      const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&sa[cb][2][o]);
+      const bf16x8 ali = *reinterpret_cast<const bf16x8*>(&sa[cb][3][o]);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         accr[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahr, bh[c], accr[r][c], 0, 0, 0);
@@ -270,6 +272,13 @@ k_beta_gemm_bf16x3(int64_t lena, int64_t lenb, int64_t ldh,
         acci[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ali, bh[c], acci[r][c], 0, 0, 0);
       }
     }
+    // step ks+1 into the other buffer (its readers finished at the last
+    // barrier), step ks+2 in flight, then one barrier
+    if (ks + 1 < nk) {
+      store_c(cb ^ 1);
+      if (ks + 2 < nk) load_c(k0 + 2 * kGK);
+    }
+    __syncthreads();
   }
   // f32 MFMA layout: item i of lane l is row 4*(l/16)+i, column l%16
 #pragma unroll
