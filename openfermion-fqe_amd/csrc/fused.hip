@@ -275,7 +275,6 @@ __device__ __forceinline__ void fused_row(
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int K = npair;
-  const int M = npair + 1;
   __bf16* dhi = reinterpret_cast<__bf16*>(smem);
   __bf16* dlo = dhi + kBfCols * kBfStride;
   RowTables rt;
