@@ -408,7 +408,7 @@ typedef struct fqehip_batch fqehip_batch;
  * a stream reads d_coeff0 and rewrites the outputs.  With group != 0 the
  * complex64 sectors that take the fused row kernel without the split
  * (hbb_hi NULL) run as one group: per Taylor step one row launch and one
- * reduction cover all of them (the rows8 row body), instead of kernels per
+ * reduction cover all of them (the rows4 row body), instead of kernels per
  * sector; the other sectors keep their own kernels on the other branches. */
 int fqehip_batch_create(int precision, int nitems,
                         const fqehip_batch_item* items, double time,

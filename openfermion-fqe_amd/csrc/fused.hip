@@ -1336,7 +1336,7 @@ k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
 // ---------------- grouped launches over the sectors of a batch --------------
 // One launch covers one Taylor step of every sector of the group: global
 // alpha row gr belongs to the last sector with row0 <= gr (a uniform
-// binary search per row), which then runs the rows4/rows8 row body.  The
+// binary search), which then runs the row body of k_fused_bf16x3.  The
 // rows of all sectors together fill the GPU where each small sector alone
 // would not.
 template <int NW, int WPE>
@@ -1589,7 +1589,8 @@ int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
 }
 
 // Grouped launches (batch code): rows of every sector of the group, then
-// their reductions; rows8 row body, LDS sized for the longest beta row.
+// their reductions; rows4 row body (two 4-wave workgroups per CU measured
+// faster here than one 8-wave one), LDS sized for the longest beta row.
 int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
                       int64_t max_lenb, int step, hipStream_t st) {
   if (rows == 0 || nsec == 0) return FQEHIP_OK;
@@ -1602,7 +1603,7 @@ int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
   // about four workgroups per CU at a time, several rounds: runs of
   // consecutive rows while the load stays balanced
   const unsigned grid = static_cast<unsigned>(rows < 4096 ? rows : 4096);
-  k_fused_group<8, 2><<<grid, 512, lds, st>>>(d_secs, nsec, rows, step);
+  k_fused_group<4, 2><<<grid, 256, lds, st>>>(d_secs, nsec, rows, step);
   FQEHIP_CHECK_LAUNCH("k_fused_group");
   return FQEHIP_OK;
 }

@@ -110,8 +110,8 @@ def test_grouped_batch_matches_oracle_and_sequential(fqe):
             for (a0, e0), (a1, e1) in zip(first, got):
                 assert rel(a1, a0) < 1e-5 and rel(e1, e0) < 1e-5
         torch.zeros(1000, device="cuda").sum().item()
-    # run_sequential takes each sector's own variant (rows4 sectors on the
-    # 4-wave kernel, the group always on the rows8 body): the beta scatter
+    # run_sequential takes each sector's own variant (rows8 sectors on the
+    # 8-wave kernel, the group always on the rows4 body): the beta scatter
     # rounds in another order, so the Taylor sums agree to mixed rounding
     bt.run_sequential()
     for (a0, e0), (a1, e1) in zip(first, bt.results()):
