@@ -1074,7 +1074,7 @@ k_split_live_f64(int norb, int compressed, int npair, int64_t lena,
 constexpr int kCF64Threads = 256;
 constexpr int kCF64Cols = 64;               // columns per chunk (4 waves x 16)
 constexpr int kCF64AS = kLiveRows + 4;      // doubles per LDS A' k-row (padding)
-constexpr int kCF64Batch = 4;               // k steps gathered per load batch
+constexpr int kCF64Batch = 7;               // k steps gathered per load batch
 constexpr int kCF64Parts = 4;               // k-step parts held in registers
 
 __global__ void __launch_bounds__(kCF64Threads, 2)
