@@ -227,9 +227,15 @@ extern "C" int fqehip_batch_create(int precision, int nitems,
       gwork += item_work(it);
     }
     const size_t bytes = hs.size() * sizeof(GroupSector);
-    if (hipMalloc(&b->d_group, bytes) != hipSuccess ||
-        hipMemcpy(b->d_group, hs.data(), bytes, hipMemcpyHostToDevice) !=
-            hipSuccess) {
+    void* dg = nullptr;
+    if (hipMalloc(&dg, bytes) != hipSuccess) {
+      release(b);
+      set_error("batch: group descriptor allocation failed");
+      return FQEHIP_ERR_OOM;
+    }
+    b->d_group = static_cast<GroupSector*>(dg);
+    if (hipMemcpy(b->d_group, hs.data(), bytes, hipMemcpyHostToDevice) !=
+        hipSuccess) {
       release(b);
       set_error("batch: group descriptor upload failed");
       return FQEHIP_ERR_HIP;
