@@ -119,6 +119,44 @@ def test_grouped_batch_matches_oracle_and_sequential(fqe):
     bt.close()
 
 
+def test_grouped_batch_without_apply(fqe):
+    """Items with d_apply_out NULL (Taylor series only) inside a group: the
+    grouped step-0 launches skip them, their Taylor sums still match the
+    oracle, and the items that do ask for H C0 get it."""
+    import ctypes
+    import torch
+    from fqe_amd import _lib, batched
+    specs = batched.random_specs(6, seed=55, norb_range=(4, 7))
+    tiles = {f"{sp.norb},{sp.nalpha},{sp.nbeta}": "rows4" for sp in specs}
+    bt = batched.BatchedTaylor(specs, 0.1, nterms=10, dtype=torch.complex64,
+                               tiles=tiles, nstreams=2, group=True)
+    items = []
+    for i, p in enumerate(bt.sectors):
+        it = p.item()
+        if i % 2:
+            it.d_apply_out = None
+            p.apply_out.fill_(7.0)
+        items.append(it)
+    arr = (_lib.BatchItem * len(items))(*items)
+    handle = ctypes.c_void_p()
+    torch.cuda.synchronize()
+    _lib.call("fqehip_batch_create", bt.prec, len(items), arr, bt.time,
+              bt.nterms, bt.nstreams, 1, _lib.stream_ptr(),
+              ctypes.byref(handle))
+    bt._batch = handle
+    assert bt.grouped() == len(specs)
+    bt.launch()
+    got = bt.results()
+    for i, ((ap, ev), sp) in enumerate(zip(got, specs)):
+        rap, rev = _oracle(sp, 0.1, 10)
+        assert rel(ev, rev) < MIXED_TOL, i
+        if i % 2:
+            assert numpy.all(ap == 7.0), i
+        else:
+            assert rel(ap, rap) < MIXED_TOL, i
+    bt.close()
+
+
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_every_variant_matches_oracle(fqe, dtype):
     """Each kernel variant of each precision (complex128: rows4, split,
