@@ -1,5 +1,5 @@
 """Edge cases and full-size properties of the apply, on every kernel path
-(FP64 fused / unfused, mixed split / row / unfused).
+(FP64 split / row / unfused, mixed split / row / unfused).
 
 * edge sectors: no electrons of one spin, full shells, one orbital, one
   determinant, compared with the oracle;
@@ -34,14 +34,17 @@ def rel(a, b):
     return numpy.linalg.norm(a - b) / max(numpy.linalg.norm(b), 1e-300)
 
 
-PATHS = [("c128", True, False), ("c128", False, False), ("c64", True, True),
-         ("c64", True, False), ("c64", False, False)]
+PATHS = [("c128", True, True), ("c128", True, False), ("c128", False, False),
+         ("c64", True, True), ("c64", True, False), ("c64", False, False)]
 
 
+# norb 15: 120 compressed pairs, past the FP64 row kernels' gather depth
+# (112), so complex128 takes the unfused kernels; complex64 keeps its split
 @pytest.mark.parametrize("dtype,fused,split", PATHS)
 @pytest.mark.parametrize("na,nb,norb", [(1, 0, 1), (1, 1, 1), (0, 0, 3),
                                         (2, 2, 2), (3, 1, 3), (0, 2, 4),
-                                        (4, 0, 4), (1, 3, 4)])
+                                        (4, 0, 4), (1, 3, 4), (1, 1, 15),
+                                        (2, 1, 15)])
 def test_edge_sectors(fqe, dtype, fused, split, na, nb, norb):
     from fqe_amd import settings
     print(f"edge {dtype} fused={fused} split={split} {na} {nb} {norb}",
@@ -51,15 +54,15 @@ def test_edge_sectors(fqe, dtype, fused, split, na, nb, norb):
     g = fo.OracleGraph(na, nb, norb)
     c = fo.synthetic_coeff(g.lena, g.lenb, rng)
     ref = fo.apply_12(c, h1, h2, g)
-    saved = settings.fused, settings.split
-    settings.fused, settings.split = fused, split
+    saved = settings.fused, settings.split, settings.split_fp64
+    settings.fused, settings.split, settings.split_fp64 = fused, split, split
     try:
         npdt = numpy.complex128 if dtype == "c128" else numpy.complex64
         d = fqe.FqeData(na, nb, norb, dtype=npdt)
         d.coeff = c
         out = d.apply((h1, h2)).coeff
     finally:
-        settings.fused, settings.split = saved
+        settings.fused, settings.split, settings.split_fp64 = saved
     tol = FP64_TOL if dtype == "c128" else MIXED_TOL
     if numpy.linalg.norm(ref) < 1e-12:
         assert numpy.linalg.norm(out) < 1e-6
