@@ -12,7 +12,8 @@ below ``epsilon``).  The method differs where the reference is quadratic:
 * the subspace matrix grows by one column per new vector, one apply each,
   instead of being rebuilt from k^2 expectation values every iteration
   (``:211-216``);
-* Gram-Schmidt runs twice (CGS2) in FP64 on the device
+* classical Gram-Schmidt in FP64 on the device, a second pass when the
+  DGKS criterion asks for it
   (``fqehip_block_dot`` / ``fqehip_block_combine``), the Rayleigh-Ritz
   eigenproblem of the k x k subspace matrix in FP64 on the host.
 
@@ -113,13 +114,22 @@ class Subspace:
         return float(numpy.sqrt(max(self._dots[0].real.item(), 0.0)))
 
     def orthonormalize(self, x: torch.Tensor) -> float:
-        """Two passes of classical Gram-Schmidt against V[:k], then
-        normalisation; returns the norm before normalisation."""
-        for _ in range(2):
-            if self.k:
+        """Classical Gram-Schmidt against V[:k] with the DGKS criterion
+        (Daniel, Gragg, Kaufman and Stewart, Math. Comp. 30 (1976)): a
+        second pass only when the first removed more than 1 - 1/sqrt(2) of
+        the norm, then normalisation; returns the norm before
+        normalisation."""
+        if self.k:
+            before = self.norm(x)
+            c = self.dots(x).clone()
+            self.combine(-c, x, beta=1.0)
+            nrm = self.norm(x)
+            if nrm < 0.7071067811865476 * before:
                 c = self.dots(x).clone()
                 self.combine(-c, x, beta=1.0)
-        nrm = self.norm(x)
+                nrm = self.norm(x)
+        else:
+            nrm = self.norm(x)
         if nrm > 0.0:
             x.mul_(1.0 / nrm)
         return nrm
