@@ -479,8 +479,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not "
-              f"supported in this build (multi-GPU sharding is designed, not "
-              f"built)", file=sys.stderr)
+              f"supported (the engine has no multi-GPU sharding)",
+              file=sys.stderr)
         sys.exit(2)
     import numpy
     import torch
