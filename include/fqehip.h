@@ -330,6 +330,14 @@ int fqehip_block_dot(int64_t n, int k, const double* d_v, int64_t ldv,
 int fqehip_block_combine(int64_t n, int k, const double* d_v, int64_t ldv,
                          const double* d_c, double beta_re, double beta_im,
                          double* d_x, void* stream);
+/* d_out[j][i] = sum_l d_c[l][j] d_v[l][i] for j < m (m <= 16; complex128
+ * rows of length n, strides ldv and ldo; d_c complex [k][m] row-major): a
+ * thick restart's new basis or images with each row read once instead of
+ * once per output (the reference rebuilds its basis in Python,
+ * src/fqe/algorithm/davidson.py:211-216). */
+int fqehip_block_combine_multi(int64_t n, int k, const double* d_v,
+                               int64_t ldv, const double* d_c, int m,
+                               double* d_out, int64_t ldo, void* stream);
 /* t[i] = -(1/(diag_i - theta) + corr_i) r[i]; |diag_i - theta| is floored
  * at eps; d_corr (FP32 [n], the learned correction) may be NULL.  The
  * reference's update is reciprocal(theta - diag) * residual

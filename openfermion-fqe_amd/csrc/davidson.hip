@@ -136,6 +136,40 @@ k_block_combine(int64_t n, int k, const double* __restrict__ v, int64_t ldv,
   reinterpret_cast<double2*>(x)[i] = make_double2(re, im);
 }
 
+// out[j][i] = sum_l c[l][j] v[l][i] for j < m <= kMultiMax: every row v[l]
+// is read once for all m outputs (a thick restart's new basis and images)
+constexpr int kMultiMax = 16;
+
+__global__ void __launch_bounds__(kThreads)
+k_block_combine_multi(int64_t n, int k, const double* __restrict__ v,
+                      int64_t ldv, const double* __restrict__ c, int m,
+                      double* __restrict__ out, int64_t ldo) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  double re[kMultiMax], im[kMultiMax];
+#pragma unroll
+  for (int j = 0; j < kMultiMax; ++j) {
+    re[j] = 0.0;
+    im[j] = 0.0;
+  }
+  for (int l = 0; l < k; ++l) {
+    const double2 vi = reinterpret_cast<const double2*>(v + 2 * l * ldv)[i];
+    const double* cl = c + 2 * static_cast<int64_t>(l) * m;
+#pragma unroll
+    for (int j = 0; j < kMultiMax; ++j) {
+      if (j < m) {
+        const double cr = cl[2 * j], ci = cl[2 * j + 1];
+        re[j] += cr * vi.x - ci * vi.y;
+        im[j] += cr * vi.y + ci * vi.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kMultiMax; ++j)
+    if (j < m)
+      reinterpret_cast<double2*>(out + 2 * j * ldo)[i] = make_double2(re[j], im[j]);
+}
+
 // t[i] = -(1 / (d_i - theta) + corr_i) r[i], |d_i - theta| floored at eps
 __global__ void __launch_bounds__(kThreads)
 k_precondition(int64_t n, const double* __restrict__ r,
@@ -209,6 +243,25 @@ extern "C" int fqehip_block_combine(int64_t n, int k, const double* d_v,
   k_block_combine<<<blocks_for(n, kThreads), kThreads, 0, as_stream(stream)>>>(
       n, k, d_v, ldv, d_c, beta_re, beta_im, d_x);
   FQEHIP_CHECK_LAUNCH("k_block_combine");
+  return FQEHIP_OK;
+}
+
+extern "C" int fqehip_block_combine_multi(int64_t n, int k, const double* d_v,
+                                          int64_t ldv, const double* d_c, int m,
+                                          double* d_out, int64_t ldo,
+                                          void* stream) {
+  clear_error();
+  FQEHIP_CHECK_ARG(n >= 0 && k >= 0 && m >= 0 && m <= kMultiMax &&
+                       (k == 0 || ldv >= n) && (m == 0 || ldo >= n),
+                   "n=%lld k=%d m=%d ldv=%lld ldo=%lld (m <= %d)",
+                   static_cast<long long>(n), k, m, static_cast<long long>(ldv),
+                   static_cast<long long>(ldo), kMultiMax);
+  if (n == 0 || m == 0) return FQEHIP_OK;
+  FQEHIP_CHECK_ARG(d_out && (k == 0 || (d_v && d_c)), "NULL pointer");
+  k_block_combine_multi<<<blocks_for(n, kThreads), kThreads, 0,
+                          as_stream(stream)>>>(n, k, d_v, ldv, d_c, m, d_out,
+                                               ldo);
+  FQEHIP_CHECK_LAUNCH("k_block_combine_multi");
   return FQEHIP_OK;
 }
 

@@ -114,6 +114,9 @@ _SIGNATURES = {
     "fqehip_block_combine": (c_int, [c_int64, c_int, c_void_p, c_int64,
                                      c_void_p, c_double, c_double, c_void_p,
                                      c_void_p]),
+    "fqehip_block_combine_multi": (c_int, [c_int64, c_int, c_void_p, c_int64,
+                                           c_void_p, c_int, c_void_p, c_int64,
+                                           c_void_p]),
     "fqehip_mlp_features": (c_int, [c_int, c_int64, c_int64, c_void_p,
                                     c_void_p, c_void_p, c_double, c_double,
                                     c_int64, c_int64, c_int64, c_void_p,

@@ -160,10 +160,19 @@ class Subspace:
             self._tmp = torch.empty((2, m, self.n), dtype=torch.complex128,
                                     device=dev)
         tmp = self._tmp
-        for j in range(m):
-            c = numpy.ascontiguousarray(coef[:, j])
-            self.combine(c, tmp[0, j])
-            self.combine(c, tmp[1, j], rows=self.av)
+        if m <= 16:
+            # all m new vectors (and images) in one pass over the rows
+            cm = torch.from_numpy(numpy.ascontiguousarray(
+                coef, dtype=numpy.complex128)).to(dev)
+            for t, rows in ((tmp[0], self.v), (tmp[1], self.av)):
+                _lib.call("fqehip_block_combine_multi", self.n, k,
+                          _lib.ptr(rows), self.n, _lib.ptr(cm), m,
+                          _lib.ptr(t), self.n, _lib.stream_ptr())
+        else:
+            for j in range(m):
+                c = numpy.ascontiguousarray(coef[:, j])
+                self.combine(c, tmp[0, j])
+                self.combine(c, tmp[1, j], rows=self.av)
         self.v[:m].copy_(tmp[0, :m])
         self.av[:m].copy_(tmp[1, :m])
         g = coef.conj().T @ self.g[:k, :k] @ coef

@@ -77,6 +77,28 @@ def test_block_dot_and_combine(fqe):
                           rtol=1e-13, atol=1e-11)
 
 
+@pytest.mark.parametrize("k,m,n", [(11, 9, 100003), (40, 16, 5000), (1, 1, 7),
+                                   (5, 3, 1)])
+def test_block_combine_multi(fqe, k, m, n):
+    """fqehip_block_combine_multi (a thick restart's new basis) against
+    numpy: out[j] = sum_l c[l][j] v[l], every output of the call."""
+    import torch
+    from fqe_amd import _lib
+    rng = numpy.random.default_rng(k * 7 + m)
+    vs = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
+    c = rng.normal(size=(k, m)) + 1j * rng.normal(size=(k, m))
+    dev = _lib.device()
+    vd = torch.from_numpy(vs).to(dev)
+    cd = torch.from_numpy(c).to(dev)
+    out = torch.full((m + 1, n), 3.0 + 3.0j, dtype=torch.complex128, device=dev)
+    _lib.call("fqehip_block_combine_multi", n, k, _lib.ptr(vd), n,
+              _lib.ptr(cd), m, _lib.ptr(out), n, _lib.stream_ptr())
+    got = out.cpu().numpy()
+    ref = c.T @ vs
+    assert numpy.max(numpy.abs(got[:m] - ref)) < 1e-12 * max(1.0, numpy.abs(ref).max())
+    assert numpy.all(got[m] == 3.0 + 3.0j)
+
+
 def test_davidson_lih_ground_state(fqe):
     """Residual-converged Davidson reaches FQE's LiH FCI energy and
     eigenvector (tests/davidson_test.py:26-40)."""
