@@ -103,3 +103,84 @@ def dense_hamiltonian(h1e: numpy.ndarray, h2e: Optional[numpy.ndarray],
             new, sign = res
             mat[basis[new], col] += sign * coef
     return mat
+
+
+# ---------------------------------------------------------------------------
+# Spin-orbital operators over every sector of one particle number
+# ---------------------------------------------------------------------------
+
+def set_basis(norb: int, nele: int):
+    """Combined basis of all (nalpha, nbeta) sectors with nalpha + nbeta =
+    nele, sectors in ascending nbeta, each in the FQE flat order.
+
+    Returns (index, blocks): index maps an occupation state (alpha on modes
+    0..norb-1, beta on modes norb..2*norb-1) to its position, blocks lists
+    ((nalpha, nbeta), offset, lena, lenb)."""
+    index = {}
+    blocks = []
+    offset = 0
+    for nbeta in range(nele + 1):
+        nalpha = nele - nbeta
+        if nalpha > norb or nbeta > norb:
+            continue
+        astr = _strings(norb, nalpha)
+        bstr = _strings(norb, nbeta)
+        for ia, sa in enumerate(astr):
+            for ib, sb in enumerate(bstr):
+                index[sa | (sb << norb)] = offset + ia * len(bstr) + ib
+        blocks.append(((nalpha, nbeta), offset, len(astr), len(bstr)))
+        offset += len(astr) * len(bstr)
+    return index, blocks
+
+
+def dense_set_operator(h1e: numpy.ndarray, h2e: Optional[numpy.ndarray],
+                       norb: int, nele: int) -> numpy.ndarray:
+    """Dense matrix of sum_pq h1[p,q] a+_p a_q + sum_pqrs h2[p,q,r,s]
+    a+_p a+_q a_r a_s over the combined basis of ``set_basis``; p, q, r, s
+    are spin-orbital modes (p < norb alpha, p >= norb beta)."""
+    index, _ = set_basis(norb, nele)
+    nmode = 2 * norb
+    terms = []
+    for p, q in itertools.product(range(nmode), repeat=2):
+        if h1e[p, q] != 0.0:
+            terms.append((h1e[p, q], ((p, True), (q, False))))
+    if h2e is not None:
+        for p, q, r, s in itertools.product(range(nmode), repeat=4):
+            if h2e[p, q, r, s] != 0.0:
+                terms.append((h2e[p, q, r, s],
+                              ((p, True), (q, True), (r, False), (s, False))))
+    mat = numpy.zeros((len(index), len(index)), dtype=numpy.complex128)
+    for state, col in index.items():
+        for coef, ops in terms:
+            res = _apply_string(state, ops)
+            if res is None:
+                continue
+            new, sign = res
+            mat[index[new], col] += sign * coef
+    return mat
+
+
+def set_sector_phase(norb: int, nele: int) -> numpy.ndarray:
+    """Diagonal phase (-1)^((nele - 1) nbeta) of each basis state of
+    ``set_basis``: the reference's FqeDataSet determinants are the
+    Jordan-Wigner ones times this phase (its spin-flip maps count occupied
+    orbitals above the orbital, reference ``src/fqe/fci_graph_set.py:
+    137-150``).  An operator M here reads P M P there."""
+    _, blocks = set_basis(norb, nele)
+    return numpy.concatenate([
+        numpy.full(lena * lenb, -1.0 if ((nele - 1) * key[1]) % 2 else 1.0)
+        for key, _, lena, lenb in blocks])
+
+
+def set_expectation(vec: numpy.ndarray, ops, norb: int, nele: int) -> complex:
+    """<vec| op string |vec> over the combined basis of ``set_basis``;
+    ``ops`` is a sequence of (mode, is_creation), applied right to left."""
+    index, _ = set_basis(norb, nele)
+    total = 0.0j
+    for state, col in index.items():
+        res = _apply_string(state, ops)
+        if res is None:
+            continue
+        new, sign = res
+        total += numpy.conj(vec[index[new]]) * sign * vec[col]
+    return total

@@ -17,6 +17,15 @@ test data.  Nothing of the reference is imported or executed:
   FCI energy -8.877719570384043 (``tests/davidson_test.py:27``) and HF
   energy -8.857341498221992 (``tests/wavefunction_test.py:537-548``) the
   Davidson tests use.
+* ``tests/unittest_data/fqe_data_set/set0202/*.npy`` (raw float64 dumps,
+  reference ``tests/unittest_data/fqe_data_set/generate_data.py:25-35``,
+  ``:54-106``, ``:108-120``; read back with ``numpy.fromfile`` as
+  ``tests/unittest_data/fqe_data_set_loader.py:26-36`` does): the
+  FqeDataSet(nele=2, norb=2) sector coefficients, the outputs of
+  apply((h1,)), apply((0, h2)) and apply((h1, h2)), and the spin-orbital
+  1- and 2-RDMs.  The spin-orbital h1 and h2 those outputs were made with
+  are rebuilt from their closed form (reference
+  ``tests/unittest_data/build_hamiltonian.py:67-119``, ``:230-286``).
 
 Run from the repository root:  python tests/golden/make_fixtures.py
 """
@@ -58,6 +67,52 @@ def fqe_data_arrays():
             arrays[f"out{suffix}_{tag}"] = (
                 raw(f"cr{tag}_{suffix}.npy") + 1j * raw(f"ci{tag}_{suffix}.npy")
             ).reshape(shape)
+    return arrays
+
+
+def spin_orbital_h(norb):
+    """h1[i,j] = 2i delta_ij + 0.02 (i + j), h2[i,j,k,l] = 0.02 (i + k)
+    (j + l) on spatial orbitals, placed on the alpha-alpha, beta-beta and
+    mixed blocks of the spin-orbital tensors with unit prefactors."""
+    idx = numpy.arange(norb, dtype=float)
+    h1 = numpy.diag(2.0 * idx) + 0.02 * (idx[:, None] + idx[None, :])
+    ik = idx[:, None] + idx[None, :]
+    h2 = 0.02 * numpy.einsum("ik,jl->ijkl", ik, ik)
+    m = 2 * norb
+    h1s = numpy.zeros((m, m))
+    h1s[:norb, :norb] = h1
+    h1s[norb:, norb:] = h1
+    h2s = numpy.zeros((m,) * 4)
+    a, b = slice(0, norb), slice(norb, m)
+    for s1, s2 in ((a, a), (a, b), (b, a), (b, b)):
+        h2s[s1, s2, s1, s2] = h2
+    return h1s, h2s
+
+
+def fqe_data_set_arrays(nele=2, norb=2):
+    from math import comb
+    base = os.path.join(REF, "unittest_data", "fqe_data_set",
+                        f"set{nele:02d}{norb:02d}")
+
+    def rd(name):
+        return numpy.fromfile(os.path.join(base, name))
+
+    arrays = {}
+    for nbeta in range(nele + 1):
+        nalpha = nele - nbeta
+        tag = f"{nalpha:02d}{nbeta:02d}{norb:02d}"
+        shape = (comb(norb, nalpha), comb(norb, nbeta))
+        arrays[f"c_{tag}"] = (rd(f"cr{tag}.npy") + 1j * rd(f"ci{tag}.npy")
+                              ).reshape(shape)
+        for suffix in ("1", "2", "12"):
+            arrays[f"out{suffix}_{tag}"] = (
+                rd(f"cr{tag}_{suffix}.npy") + 1j * rd(f"ci{tag}_{suffix}.npy")
+            ).reshape(shape)
+    m = 2 * norb
+    for order in (1, 2):
+        arrays[f"rdm{order}"] = (rd(f"dr{order}.npy") + 1j * rd(f"di{order}.npy")
+                                 ).reshape((m,) * (2 * order))
+    arrays["h1"], arrays["h2"] = spin_orbital_h(norb)
     return arrays
 
 
@@ -113,6 +168,8 @@ def lih_arrays():
 def main():
     numpy.savez(os.path.join(OUT, "fqe_data_golden.npz"), **fqe_data_arrays())
     numpy.savez(os.path.join(OUT, "lih_energy.npz"), **lih_arrays())
+    numpy.savez(os.path.join(OUT, "fqe_data_set_golden.npz"),
+                **fqe_data_set_arrays())
     with open(os.path.join(OUT, "literal_tables.json"), "w") as f:
         json.dump(literal_tables(), f, indent=1)
 
