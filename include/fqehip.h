@@ -439,6 +439,36 @@ int64_t fqehip_gram_workspace_bytes(int64_t n, int k);
 int fqehip_gram(int64_t n, int k, const double* d_a, const double* d_b,
                 double* d_out, double* d_workspace, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Spin-orbital apply on a full particle-number set (FqeDataSet): every
+ * sector (nalpha, nbeta) with nalpha + nbeta = nele and both <= norb, in
+ * ascending nbeta, concatenated into one complex128 vector of
+ * fqehip_set_length(norb, nele) elements (each sector row-major
+ * lena x lenb).  Spin-orbital index p < norb is alpha orbital p, p >= norb
+ * beta orbital p - norb; M = 2 norb; norb <= 16.  Complex arrays are
+ * interleaved complex128.
+ *
+ * fqehip_set_dvec: d_dvec[p][q] = E_pq C, complex128 [M][M][L].  Replaces
+ * FqeDataSet._calculate_dvec_with_coeff (src/fqe/fqe_data_set.py:737-796).
+ *
+ * fqehip_set_apply with d_v == NULL: out = sum_pq h1[p][q] E_pq C
+ * (FqeDataSet._apply1, src/fqe/fqe_data_set.py:153-181); no workspace.
+ * With d_v: out = sum_pq h1[p][q] D[pq] + sum_pq E_pq (V . D)[pq], where the
+ * caller passes the dvec-form tensors V[i][j][k][l] = -h2[i][k][j][l] and
+ * h1' = h1 - sum_k V[:, k, k, :] (FqeDataSet._apply12,
+ * src/fqe/fqe_data_set.py:183-209); d_workspace holds
+ * fqehip_set_workspace_bytes(norb, nele) bytes.  Spin-flip signs are the
+ * reference's (src/fqe/fci_graph_set.py:137-150).  Gathers only: the same
+ * bits on every run.  d_out must not alias d_coeff.
+ * ------------------------------------------------------------------- */
+int64_t fqehip_set_length(int norb, int nele);
+int64_t fqehip_set_workspace_bytes(int norb, int nele);
+int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
+                    double* d_dvec, void* stream);
+int fqehip_set_apply(int norb, int nele, const double* d_h1,
+                     const double* d_v, const double* d_coeff, double* d_out,
+                     double* d_workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

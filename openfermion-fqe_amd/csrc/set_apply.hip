@@ -1,0 +1,332 @@
+// Spin-orbital apply on a full particle-number set of sectors (FqeDataSet).
+//
+// The set holds every (nalpha, nbeta) sector with nalpha + nbeta = nele and
+// both counts <= norb, in ascending nbeta, concatenated into one complex128
+// vector of length L (each sector row-major lena x lenb).  Spin-orbital
+// index p < norb is alpha orbital p, p >= norb beta orbital p - norb, and
+// M = 2 norb.  For a target determinant I and an ordered pair (p, q) there
+// is at most one source determinant J with <I|a+_p a_q|J> != 0; every
+// kernel here is a gather over that J, so no kernel writes to a location
+// another thread writes and no atomics are used.
+//
+// Signs follow the reference: within one spin (-1)^(occupied orbitals
+// strictly between i and j) (src/fqe/fci_graph.py:204-244); across spins
+// (-1)^nalpha(J) times the parities of the sector-to-sector maps, which
+// count the occupied orbitals above the orbital
+// (src/fqe/fqe_data_set.py:737-796, src/fqe/fci_graph_set.py:137-150).
+//
+// Kernels (grid: one thread per target determinant, 256 per block):
+//   k_set_dvec  D[pq][I] = sum_J <I|E_pq|J> C[J]            (M^2 L writes)
+//   k_set_vd    Y[pq][g] = sum_rs V[pq][rs] D[rs][g]        (M^4 L FMAs)
+//   k_set_out   out[I]   = sum_pq hd[pq] D[pq][I]
+//                        + sum_pq hg[pq] <I|E_pq|J> G[pq][J]
+// with (hd, D, hg, G) = (h1', D, 1, Y) for the 1+2-body apply and
+// (-, -, h1, C) for the 1-body apply.
+#include "fqehip_common.h"
+
+namespace fqehip {
+namespace {
+
+constexpr int kSetMaxNorb = 16;
+constexpr int kSetMaxSec = kSetMaxNorb + 1;
+constexpr int kSetBlock = 256;
+
+struct SetDesc {
+  int64_t total;                 // L
+  int norb, nele, nsec, nbmin;   // sector s has nbeta = nbmin + s
+  int64_t off[kSetMaxSec + 1];   // sector offsets, off[nsec] = L
+  int32_t lenb[kSetMaxSec];
+  int32_t binom[kSetMaxNorb + 1][kSetMaxNorb + 1];
+};
+
+__device__ __forceinline__ int popc(uint32_t x) { return __popc(x); }
+
+// Address of an nele-electron string in the reference's order (ascending
+// occupation tuples in lexicographic order):
+// C(n, k) - 1 - sum_t C(n - 1 - c_t, k + 1 - t), occupied c_1 < ... < c_k.
+__device__ __forceinline__ int32_t lex_rank(uint32_t s, int n, int k,
+                                            const SetDesc& d) {
+  int32_t r = d.binom[n][k] - 1;
+  int t = 1;
+  while (s) {
+    int c = __ffs(s) - 1;
+    s &= s - 1;
+    r -= d.binom[n - 1 - c][k + 1 - t];
+    ++t;
+  }
+  return r;
+}
+
+// Inverse of lex_rank: the mirrored set has colex rank C(n,k) - 1 - r.
+__device__ __forceinline__ uint32_t lex_unrank(int32_t r, int n, int k,
+                                               const SetDesc& d) {
+  int32_t rem = d.binom[n][k] - 1 - r;
+  uint32_t s = 0;
+  int x = n - 1;
+  for (int i = k; i >= 1; --i) {
+    while (d.binom[x][i] > rem) --x;   // stops at x = i - 1 at the latest
+    s |= 1u << (n - 1 - x);
+    rem -= d.binom[x][i];
+    --x;
+  }
+  return s;
+}
+
+__device__ __forceinline__ int bits_between32(uint32_t s, int i, int j) {
+  int lo = i < j ? i : j, hi = i < j ? j : i;
+  uint32_t mask = ((1u << hi) - 1u) & ~((2u << lo) - 1u);
+  return popc(s & mask);
+}
+
+// Target determinant of sector `sec` with strings (a, b).  For pair (p, q)
+// returns false when <I|E_pq|J> = 0 for every J; otherwise the flat index
+// of J in the set vector and its sign.
+__device__ __forceinline__ bool set_source(int p, int q, int sec, int na,
+                                           int nb, uint32_t a, uint32_t b,
+                                           const SetDesc& d, int64_t* jflat,
+                                           double* sign) {
+  const int n = d.norb;
+  if (p < n && q < n) {            // alpha a+_i a_j, same sector
+    const int i = p, j = q;
+    if (!((a >> i) & 1u)) return false;
+    if (i != j && ((a >> j) & 1u)) return false;
+    const uint32_t src = (a ^ (1u << i)) | (1u << j);
+    *sign = (bits_between32(a, i, j) & 1) ? -1.0 : 1.0;
+    const int32_t ia = lex_rank(src, n, na, d);
+    const int64_t lenb = d.lenb[sec];
+    *jflat = d.off[sec] + ia * lenb + lex_rank(b, n, nb, d);
+    return true;
+  }
+  if (p >= n && q >= n) {          // beta a+_i a_j, same sector
+    const int i = p - n, j = q - n;
+    if (!((b >> i) & 1u)) return false;
+    if (i != j && ((b >> j) & 1u)) return false;
+    const uint32_t src = (b ^ (1u << i)) | (1u << j);
+    *sign = (bits_between32(b, i, j) & 1) ? -1.0 : 1.0;
+    const int64_t lenb = d.lenb[sec];
+    *jflat = d.off[sec] + lex_rank(a, n, na, d) * lenb + lex_rank(src, n, nb, d);
+    return true;
+  }
+  if (p < n) {                     // a+_{i alpha} a_{j beta}: J in (na-1, nb+1)
+    const int i = p, j = q - n;
+    if (!((a >> i) & 1u) || ((b >> j) & 1u)) return false;
+    const int ssec = sec + 1;
+    if (ssec >= d.nsec) return false;
+    const uint32_t sa = a ^ (1u << i), sb = b | (1u << j);
+    const int par = (na - 1) + popc(a >> (i + 1)) + popc(b >> (j + 1));
+    *sign = (par & 1) ? -1.0 : 1.0;
+    const int64_t lenb = d.lenb[ssec];
+    *jflat = d.off[ssec] + lex_rank(sa, n, na - 1, d) * lenb +
+             lex_rank(sb, n, nb + 1, d);
+    return true;
+  }
+  {                                // a+_{i beta} a_{j alpha}: J in (na+1, nb-1)
+    const int i = p - n, j = q;
+    if (!((b >> i) & 1u) || ((a >> j) & 1u)) return false;
+    const int ssec = sec - 1;
+    if (ssec < 0) return false;
+    const uint32_t sa = a | (1u << j), sb = b ^ (1u << i);
+    const int par = na + popc(a >> (j + 1)) + popc(b >> (i + 1));
+    *sign = (par & 1) ? -1.0 : 1.0;
+    const int64_t lenb = d.lenb[ssec];
+    *jflat = d.off[ssec] + lex_rank(sa, n, na + 1, d) * lenb +
+             lex_rank(sb, n, nb - 1, d);
+    return true;
+  }
+}
+
+// Sector, counts and strings of flat index g (g < L).
+__device__ __forceinline__ void locate(int64_t g, const SetDesc& d, int* sec,
+                                       int* na, int* nb, uint32_t* a,
+                                       uint32_t* b) {
+  int s = 0;
+  while (s + 1 < d.nsec && g >= d.off[s + 1]) ++s;
+  *sec = s;
+  *nb = d.nbmin + s;
+  *na = d.nele - *nb;
+  const int64_t r = g - d.off[s];
+  const int64_t lenb = d.lenb[s];
+  *a = lex_unrank(static_cast<int32_t>(r / lenb), d.norb, *na, d);
+  *b = lex_unrank(static_cast<int32_t>(r % lenb), d.norb, *nb, d);
+}
+
+__global__ void __launch_bounds__(kSetBlock)
+k_set_dvec(SetDesc d, int m2, const double2* __restrict__ c,
+           double2* __restrict__ dvec) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
+  if (g >= d.total) return;
+  int sec, na, nb;
+  uint32_t a, b;
+  locate(g, d, &sec, &na, &nb, &a, &b);
+  const int m = 2 * d.norb;
+  for (int pq = 0; pq < m2; ++pq) {
+    int64_t j;
+    double sg;
+    double2 v = make_double2(0.0, 0.0);
+    if (set_source(pq / m, pq % m, sec, na, nb, a, b, d, &j, &sg)) {
+      const double2 x = c[j];
+      v = make_double2(sg * x.x, sg * x.y);
+    }
+    dvec[static_cast<int64_t>(pq) * d.total + g] = v;
+  }
+}
+
+// Y[pq][g] = sum_rs V[pq][rs] D[rs][g]; grid (blocks over g, pq).
+__global__ void __launch_bounds__(kSetBlock)
+k_set_vd(int64_t total, int m2, const double2* __restrict__ v,
+         const double2* __restrict__ dvec, double2* __restrict__ y) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
+  const int pq = blockIdx.y;
+  if (g >= total) return;
+  const double2* vrow = v + static_cast<int64_t>(pq) * m2;
+  double re = 0.0, im = 0.0;
+  for (int rs = 0; rs < m2; ++rs) {
+    const double2 h = vrow[rs];
+    const double2 x = dvec[static_cast<int64_t>(rs) * total + g];
+    re += h.x * x.x - h.y * x.y;
+    im += h.x * x.y + h.y * x.x;
+  }
+  y[static_cast<int64_t>(pq) * total + g] = make_double2(re, im);
+}
+
+// out[I] = sum_pq hd[pq] D[pq][I] + sum_pq hg[pq] sign G[pq * gstride + J].
+// hd / dvec may be null (no direct term); hg null means all ones.
+__global__ void __launch_bounds__(kSetBlock)
+k_set_out(SetDesc d, int m2, const double2* __restrict__ hd,
+          const double2* __restrict__ dvec, const double2* __restrict__ hg,
+          const double2* __restrict__ gsrc, int64_t gstride,
+          double2* __restrict__ out) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
+  if (g >= d.total) return;
+  int sec, na, nb;
+  uint32_t a, b;
+  locate(g, d, &sec, &na, &nb, &a, &b);
+  const int m = 2 * d.norb;
+  double re = 0.0, im = 0.0;
+  for (int pq = 0; pq < m2; ++pq) {
+    if (dvec != nullptr) {
+      const double2 h = hd[pq];
+      const double2 x = dvec[static_cast<int64_t>(pq) * d.total + g];
+      re += h.x * x.x - h.y * x.y;
+      im += h.x * x.y + h.y * x.x;
+    }
+    int64_t j;
+    double sg;
+    if (set_source(pq / m, pq % m, sec, na, nb, a, b, d, &j, &sg)) {
+      const double2 x = gsrc[static_cast<int64_t>(pq) * gstride + j];
+      double2 h = make_double2(sg, 0.0);
+      if (hg != nullptr) h = make_double2(sg * hg[pq].x, sg * hg[pq].y);
+      re += h.x * x.x - h.y * x.y;
+      im += h.x * x.y + h.y * x.x;
+    }
+  }
+  out[g] = make_double2(re, im);
+}
+
+int64_t host_binom(int n, int k) {
+  if (k < 0 || n < 0 || k > n) return 0;
+  int64_t r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+int make_desc(int norb, int nele, SetDesc* d) {
+  FQEHIP_CHECK_ARG(norb >= 1 && norb <= kSetMaxNorb,
+                   "set apply: norb %d outside 1..%d", norb, kSetMaxNorb);
+  FQEHIP_CHECK_ARG(nele >= 0 && nele <= 2 * norb,
+                   "set apply: nele %d outside 0..%d", nele, 2 * norb);
+  *d = SetDesc{};
+  d->norb = norb;
+  d->nele = nele;
+  d->nbmin = nele > norb ? nele - norb : 0;
+  const int nbmax = nele < norb ? nele : norb;
+  d->nsec = nbmax - d->nbmin + 1;
+  for (int n = 0; n <= kSetMaxNorb; ++n)
+    for (int k = 0; k <= kSetMaxNorb; ++k)
+      d->binom[n][k] = static_cast<int32_t>(host_binom(n, k));
+  int64_t off = 0;
+  for (int s = 0; s < d->nsec; ++s) {
+    const int nb = d->nbmin + s, na = nele - nb;
+    d->off[s] = off;
+    d->lenb[s] = static_cast<int32_t>(host_binom(norb, nb));
+    off += host_binom(norb, na) * host_binom(norb, nb);
+  }
+  d->off[d->nsec] = off;
+  d->total = off;
+  return FQEHIP_OK;
+}
+
+}  // namespace
+}  // namespace fqehip
+
+using namespace fqehip;
+
+extern "C" int64_t fqehip_set_length(int norb, int nele) {
+  SetDesc d;
+  if (make_desc(norb, nele, &d) != FQEHIP_OK) return -1;
+  return d.total;
+}
+
+extern "C" int64_t fqehip_set_workspace_bytes(int norb, int nele) {
+  SetDesc d;
+  if (make_desc(norb, nele, &d) != FQEHIP_OK) return -1;
+  const int64_t m2 = 4LL * norb * norb;
+  return 2 * m2 * d.total * static_cast<int64_t>(sizeof(double2));
+}
+
+extern "C" int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
+                               double* d_dvec, void* stream) {
+  clear_error();
+  SetDesc d;
+  if (int rc = make_desc(norb, nele, &d)) return rc;
+  FQEHIP_CHECK_ARG(d_coeff != nullptr && d_dvec != nullptr,
+                   "fqehip_set_dvec: null pointer");
+  if (d.total == 0) return FQEHIP_OK;
+  const int m2 = 4 * norb * norb;
+  k_set_dvec<<<blocks_for(d.total, kSetBlock), kSetBlock, 0,
+               as_stream(stream)>>>(d, m2,
+                                    reinterpret_cast<const double2*>(d_coeff),
+                                    reinterpret_cast<double2*>(d_dvec));
+  FQEHIP_CHECK_LAUNCH("k_set_dvec");
+  return FQEHIP_OK;
+}
+
+extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
+                                const double* d_v, const double* d_coeff,
+                                double* d_out, double* d_workspace,
+                                void* stream) {
+  clear_error();
+  SetDesc d;
+  if (int rc = make_desc(norb, nele, &d)) return rc;
+  FQEHIP_CHECK_ARG(d_h1 != nullptr && d_coeff != nullptr && d_out != nullptr,
+                   "fqehip_set_apply: null pointer");
+  FQEHIP_CHECK_ARG(d_v == nullptr || d_workspace != nullptr,
+                   "fqehip_set_apply: the two-body apply needs a workspace");
+  FQEHIP_CHECK_ARG(d_coeff != d_out,
+                   "fqehip_set_apply: output must not alias the input");
+  if (d.total == 0) return FQEHIP_OK;
+  const int m2 = 4 * norb * norb;
+  const unsigned nblk = blocks_for(d.total, kSetBlock);
+  hipStream_t s = as_stream(stream);
+  const double2* h1 = reinterpret_cast<const double2*>(d_h1);
+  const double2* c = reinterpret_cast<const double2*>(d_coeff);
+  double2* out = reinterpret_cast<double2*>(d_out);
+  if (d_v == nullptr) {
+    k_set_out<<<nblk, kSetBlock, 0, s>>>(d, m2, nullptr, nullptr, h1, c, 0,
+                                         out);
+    FQEHIP_CHECK_LAUNCH("k_set_out");
+    return FQEHIP_OK;
+  }
+  FQEHIP_CHECK_ARG(m2 <= 65535, "fqehip_set_apply: too many pairs");
+  double2* dv = reinterpret_cast<double2*>(d_workspace);
+  double2* y = dv + static_cast<int64_t>(m2) * d.total;
+  k_set_dvec<<<nblk, kSetBlock, 0, s>>>(d, m2, c, dv);
+  FQEHIP_CHECK_LAUNCH("k_set_dvec");
+  k_set_vd<<<dim3(nblk, m2), kSetBlock, 0, s>>>(
+      d.total, m2, reinterpret_cast<const double2*>(d_v), dv, y);
+  FQEHIP_CHECK_LAUNCH("k_set_vd");
+  k_set_out<<<nblk, kSetBlock, 0, s>>>(d, m2, h1, dv, nullptr, y, d.total,
+                                       out);
+  FQEHIP_CHECK_LAUNCH("k_set_out");
+  return FQEHIP_OK;
+}

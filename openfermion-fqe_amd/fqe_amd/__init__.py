@@ -13,6 +13,7 @@ Drop-in subset of the ``fqe`` namespace for that path (reference
 from . import settings
 from .fci_graph import FciGraph
 from .fqe_data import FqeData
+from .fqe_data_set import FqeDataSet
 from .hamiltonians import Hamiltonian, RestrictedHamiltonian
 from .util import vdot, alpha_beta_electrons
 from .wavefunction import Wavefunction as _Wavefunction

@@ -117,6 +117,11 @@ _SIGNATURES = {
     "fqehip_block_combine_multi": (c_int, [c_int64, c_int, c_void_p, c_int64,
                                            c_void_p, c_int, c_void_p, c_int64,
                                            c_void_p]),
+    "fqehip_set_length": (c_int64, [c_int, c_int]),
+    "fqehip_set_workspace_bytes": (c_int64, [c_int, c_int]),
+    "fqehip_set_dvec": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "fqehip_set_apply": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
     "fqehip_mlp_features": (c_int, [c_int, c_int64, c_int64, c_void_p,
                                     c_void_p, c_void_p, c_double, c_double,
                                     c_int64, c_int64, c_int64, c_void_p,
@@ -192,7 +197,7 @@ def ptr(t) -> int:
 def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
     if name.endswith(("_count", "_bytes", "_supported", "_ld", "_floats",
-                      "_nodes", "_grouped", "_destroy")):
+                      "_nodes", "_grouped", "_destroy", "_length")):
         return rc
     if rc != OK:
         msg = lib().fqehip_last_error().decode(errors="replace")

@@ -58,6 +58,37 @@ def test_argument_errors_are_reported_without_gpu_work():
     assert lib.fqehip_norm_partials_count(257) == 2
 
 
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libfqehip.so not built")
+def test_set_sizes_and_argument_errors_without_gpu_work():
+    """The spin-orbital set entry points size the concatenated sectors as
+    sum over nbeta of C(norb, nele - nbeta) C(norb, nbeta) = C(2 norb, nele)
+    and reject bad sizes and pointers on the host."""
+    import math
+    import torch  # noqa: F401
+    from fqe_amd import _lib
+    lib = _lib.lib()
+    for norb in range(1, 9):
+        for nele in range(0, 2 * norb + 1):
+            n = lib.fqehip_set_length(norb, nele)
+            assert n == math.comb(2 * norb, nele)
+            assert lib.fqehip_set_workspace_bytes(norb, nele) == (
+                2 * (2 * norb) ** 2 * n * 16)
+    assert lib.fqehip_set_length(17, 2) == -1
+    assert lib.fqehip_set_length(3, 7) == -1
+    rc = lib.fqehip_set_apply(17, 2, None, None, None, None, None, None)
+    assert rc == _lib.ERR_INVALID
+    assert b"norb" in lib.fqehip_last_error()
+    rc = lib.fqehip_set_apply(3, 3, None, None, None, None, None, None)
+    assert rc == _lib.ERR_INVALID
+    assert b"null" in lib.fqehip_last_error()
+    rc = lib.fqehip_set_apply(3, 3, 8, 8, 16, 32, None, None)
+    assert rc == _lib.ERR_INVALID
+    assert b"workspace" in lib.fqehip_last_error()
+    rc = lib.fqehip_set_apply(3, 3, 8, None, 16, 16, None, None)
+    assert rc == _lib.ERR_INVALID
+    assert b"alias" in lib.fqehip_last_error()
+
+
 def _header_prototypes():
     """name -> number of parameters of each declared entry point."""
     text = open(HEADER).read()
