@@ -78,6 +78,14 @@ def parse():
                          "Hamiltonians, then solve held-out norb=14 7a7b "
                          "ones (seed 4) to a 1e-8 residual with and without "
                          "it; prints its own JSON line")
+    ap.add_argument("--set", action="store_true",
+                    help="run the spin-orbital FqeDataSet workload instead "
+                         "(every sector of nele electrons on norb orbitals, "
+                         "random Hermitian spin-mixing h1/h2, seed 5, "
+                         "complex128 1+2-body apply) and print its own JSON "
+                         "line")
+    ap.add_argument("--set-norb", type=int, default=10,
+                    help="set workload: norb (nele = norb)")
     ap.add_argument("--train", type=int, default=120,
                     help="training Hamiltonians (norb 8 and 9)")
     ap.add_argument("--heldout", type=int, default=3,
@@ -438,6 +446,117 @@ def host_boundary_rate(coeff, h1, h2, norb, na, nb, reps=3):
     return reps / sum(times)
 
 
+def set_main(args):
+    """FqeDataSet 1+2-body apply (complex128) through fqehip_set_apply, with
+    the three kernels timed on their own; the oracle's apply of the same
+    workload is the CPU baseline and the parity check."""
+    import math
+    import numpy
+    import torch
+    from fqe_amd import _lib as L
+    norb = args.set_norb
+    nele = norb
+    m = 2 * norb
+    m2 = m * m
+    n = int(L.call("fqehip_set_length", norb, nele))
+    assert n == math.comb(2 * norb, nele)
+    rng = numpy.random.default_rng(5)
+    h1 = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    h2 = rng.standard_normal((m,) * 4) + 1j * rng.standard_normal((m,) * 4)
+    h1 = h1 + h1.conj().T
+    h2 = h2 + h2.conj().transpose(3, 2, 1, 0)
+    coeff = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    coeff /= numpy.linalg.norm(coeff)
+    v = -numpy.moveaxis(h2, 1, 2)
+    h1p = h1 - numpy.einsum("ikkj->ij", v)
+    dev = L.device()
+    tc = torch.from_numpy(coeff).to(dev)
+    th1 = torch.from_numpy(numpy.ascontiguousarray(h1p)).to(dev)
+    tv = torch.from_numpy(numpy.ascontiguousarray(v.reshape(m2, m2))).to(dev)
+    out = torch.empty_like(tc)
+    ws = torch.empty(int(L.call("fqehip_set_workspace_bytes", norb, nele)),
+                     dtype=torch.uint8, device=dev)
+    d_ptr = ws.data_ptr()
+    st = L.stream_ptr()
+
+    def apply():
+        L.call("fqehip_set_apply", norb, nele, L.ptr(th1), L.ptr(tv),
+               L.ptr(tc), L.ptr(out), d_ptr, st)
+
+    def timed(fn, steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(steps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        return ev[0].elapsed_time(ev[1]) / steps
+
+    for _ in range(args.warmup):
+        apply()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        apply()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_apply = timed(apply, args.steps)
+    t_dvec = timed(lambda: L.call("fqehip_set_dvec", norb, nele, L.ptr(tc),
+                                  d_ptr, st), args.steps)
+    t_vd = timed(lambda: L.call("fqehip_set_contract", norb, nele, L.ptr(tv),
+                                d_ptr, st), args.steps)
+    apply()
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    flops_vd = 8.0 * m2 * m2 * n
+    achieved = flops_vd / (t_vd * 1e-3) / 1e12
+    line = {
+        "metric": "FqeDataSet 1+2-body applies/s (spin-orbital, complex128)",
+        "value": round(args.steps / wall, 3), "unit": "applies/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic",
+        "config": {"workload": (f"FqeDataSet nele={nele} norb={norb}: every "
+                                f"sector, {n} determinants, random Hermitian "
+                                f"spin-mixing h1/h2, seed 5")},
+        "kernels_ms": {"k_set_dvec": round(t_dvec, 4),
+                       "contract (k_set_expand_v + k_gemm_f64)": round(t_vd, 4),
+                       "k_set_out": round(t_apply - t_dvec - t_vd, 4),
+                       "apply": round(t_apply, 4)},
+        "roofline": {"kernel": "k_gemm_f64 (set contraction)", "bound": "mfma",
+                     "achieved": round(achieved, 3), "peak": 78.6,
+                     "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4),
+                     "traffic": None,
+                     "note": ("8 M^4 L algorithmic FP64 flops per "
+                              "contraction (real block GEMM, m = k = 2 M^2, "
+                              "L columns); time includes the V expansion")},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import fqe_set_oracle as so
+        cores = len(os.sched_getaffinity(0))
+        sg = so.SetGraph(nele, norb)
+        coeffs, off = {}, 0
+        for k in sg.keys:
+            sz = sg.shape(k)[0] * sg.shape(k)[1]
+            coeffs[k] = coeff[off:off + sz].reshape(sg.shape(k))
+            off += sz
+        t1 = time.perf_counter()
+        want = so.set_apply(coeffs, (h1, h2), sg)
+        dt = time.perf_counter() - t1
+        want = numpy.concatenate([want[k].ravel() for k in sg.keys])
+        line["oracle_rel_err"] = float(numpy.linalg.norm(got - want) /
+                                       numpy.linalg.norm(want))
+        line["cpu_baseline"] = {
+            "value": round(1.0 / dt, 5), "unit": "applies/s",
+            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
+            "kind": "port",
+            "sample": (f"1 apply of the same workload by the NumPy oracle "
+                       f"(dense string matrices, BLAS threads), {dt:.2f} s")}
+    print(json.dumps(line), flush=True)
+
+
 def cpu_baseline(coeff, h1, h2, norb, na, nb):
     """One apply with the reference's compiled lm kernels on all host
     cores (complex128)."""
@@ -476,6 +595,9 @@ def main():
         return
     if args.davidson and args.gpus == 1:
         davidson_main(args)
+        return
+    if args.set and args.gpus == 1:
+        set_main(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not "

@@ -448,7 +448,8 @@ int fqehip_gram(int64_t n, int k, const double* d_a, const double* d_b,
  * beta orbital p - norb; M = 2 norb; norb <= 16.  Complex arrays are
  * interleaved complex128.
  *
- * fqehip_set_dvec: d_dvec[p][q] = E_pq C, complex128 [M][M][L].  Replaces
+ * fqehip_set_dvec: D[pq] = E_pq C for pq = p*M + q, planar doubles
+ * [M*M][2][L] (real parts, then imaginary parts, of each pair).  Replaces
  * FqeDataSet._calculate_dvec_with_coeff (src/fqe/fqe_data_set.py:737-796).
  *
  * fqehip_set_apply with d_v == NULL: out = sum_pq h1[p][q] E_pq C
@@ -468,6 +469,14 @@ int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
 int fqehip_set_apply(int norb, int nele, const double* d_h1,
                      const double* d_v, const double* d_coeff, double* d_out,
                      double* d_workspace, void* stream);
+/* The contraction stage of the two-body apply on its own, on the
+ * workspace of fqehip_set_apply: D (planar [M^2][2][L], as fqehip_set_dvec
+ * writes it) at its start, Y[pq] = sum_rs V[pq][rs] D[rs] (same layout)
+ * after it, as one real FP64 matrix-core GEMM with the block form
+ * [[Re V, -Im V], [Im V, Re V]] (the V . D tensordot of
+ * src/fqe/fqe_data_set.py:204-205). */
+int fqehip_set_contract(int norb, int nele, const double* d_v,
+                        double* d_workspace, void* stream);
 
 #ifdef __cplusplus
 }

@@ -15,13 +15,17 @@
 // count the occupied orbitals above the orbital
 // (src/fqe/fqe_data_set.py:737-796, src/fqe/fci_graph_set.py:137-150).
 //
-// Kernels (grid: one thread per target determinant, 256 per block):
-//   k_set_dvec  D[pq][I] = sum_J <I|E_pq|J> C[J]            (M^2 L writes)
-//   k_set_vd    Y[pq][g] = sum_rs V[pq][rs] D[rs][g]        (M^4 L FMAs)
+// Kernels:
+//   k_set_dvec  D[pq][I] = sum_J <I|E_pq|J> C[J]   (one thread per I)
+//   k_set_expand_v  the real block form [[Vr, -Vi], [Vi, Vr]] of V
+//   k_gemm_f64  Y = V D as one real FP64 matrix-core GEMM, m = k = 2 M^2
+//               (fqehip_pair_gemm)
 //   k_set_out   out[I]   = sum_pq hd[pq] D[pq][I]
 //                        + sum_pq hg[pq] <I|E_pq|J> G[pq][J]
 // with (hd, D, hg, G) = (h1', D, 1, Y) for the 1+2-body apply and
-// (-, -, h1, C) for the 1-body apply.
+// (-, -, h1, C) for the 1-body apply.  D and Y are planar per pair:
+// row 2 pq holds the real parts, row 2 pq + 1 the imaginary parts, L
+// doubles each, so the GEMM sees them as real 2 M^2 x L matrices.
 #include "fqehip_common.h"
 
 namespace fqehip {
@@ -152,7 +156,7 @@ __device__ __forceinline__ void locate(int64_t g, const SetDesc& d, int* sec,
 
 __global__ void __launch_bounds__(kSetBlock)
 k_set_dvec(SetDesc d, int m2, const double2* __restrict__ c,
-           double2* __restrict__ dvec) {
+           double* __restrict__ dvec) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
   if (g >= d.total) return;
   int sec, na, nb;
@@ -162,40 +166,38 @@ k_set_dvec(SetDesc d, int m2, const double2* __restrict__ c,
   for (int pq = 0; pq < m2; ++pq) {
     int64_t j;
     double sg;
-    double2 v = make_double2(0.0, 0.0);
+    double re = 0.0, im = 0.0;
     if (set_source(pq / m, pq % m, sec, na, nb, a, b, d, &j, &sg)) {
       const double2 x = c[j];
-      v = make_double2(sg * x.x, sg * x.y);
+      re = sg * x.x;
+      im = sg * x.y;
     }
-    dvec[static_cast<int64_t>(pq) * d.total + g] = v;
+    dvec[static_cast<int64_t>(2 * pq) * d.total + g] = re;
+    dvec[static_cast<int64_t>(2 * pq + 1) * d.total + g] = im;
   }
 }
 
-// Y[pq][g] = sum_rs V[pq][rs] D[rs][g]; grid (blocks over g, pq).
+// a[(2 pq + u) * 2 m2 + 2 rs + w] = [[Vr, -Vi], [Vi, Vr]][u][w] of
+// V[pq][rs] (complex128 [m2][m2]); one thread per element of a.
 __global__ void __launch_bounds__(kSetBlock)
-k_set_vd(int64_t total, int m2, const double2* __restrict__ v,
-         const double2* __restrict__ dvec, double2* __restrict__ y) {
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
-  const int pq = blockIdx.y;
-  if (g >= total) return;
-  const double2* vrow = v + static_cast<int64_t>(pq) * m2;
-  double re = 0.0, im = 0.0;
-  for (int rs = 0; rs < m2; ++rs) {
-    const double2 h = vrow[rs];
-    const double2 x = dvec[static_cast<int64_t>(rs) * total + g];
-    re += h.x * x.x - h.y * x.y;
-    im += h.x * x.y + h.y * x.x;
-  }
-  y[static_cast<int64_t>(pq) * total + g] = make_double2(re, im);
+k_set_expand_v(int m2, const double2* __restrict__ v, double* __restrict__ a) {
+  const int64_t n2 = 2LL * m2;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
+  if (e >= n2 * n2) return;
+  const int64_t row = e / n2, col = e % n2;
+  const double2 h = v[(row >> 1) * m2 + (col >> 1)];
+  const int u = static_cast<int>(row & 1), w = static_cast<int>(col & 1);
+  a[e] = (u == w) ? h.x : (u == 0 ? -h.y : h.y);
 }
 
-// out[I] = sum_pq hd[pq] D[pq][I] + sum_pq hg[pq] sign G[pq * gstride + J].
+// out[I] = sum_pq hd[pq] D[pq][I] + sum_pq hg[pq] sign G[pq][J].
 // hd / dvec may be null (no direct term); hg null means all ones.
+// G is the planar Y (kPlanarG) or the interleaved coefficients C.
+template <bool kPlanarG>
 __global__ void __launch_bounds__(kSetBlock)
 k_set_out(SetDesc d, int m2, const double2* __restrict__ hd,
-          const double2* __restrict__ dvec, const double2* __restrict__ hg,
-          const double2* __restrict__ gsrc, int64_t gstride,
-          double2* __restrict__ out) {
+          const double* __restrict__ dvec, const double2* __restrict__ hg,
+          const double* __restrict__ gsrc, double2* __restrict__ out) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
   if (g >= d.total) return;
   int sec, na, nb;
@@ -206,14 +208,22 @@ k_set_out(SetDesc d, int m2, const double2* __restrict__ hd,
   for (int pq = 0; pq < m2; ++pq) {
     if (dvec != nullptr) {
       const double2 h = hd[pq];
-      const double2 x = dvec[static_cast<int64_t>(pq) * d.total + g];
-      re += h.x * x.x - h.y * x.y;
-      im += h.x * x.y + h.y * x.x;
+      const double xr = dvec[static_cast<int64_t>(2 * pq) * d.total + g];
+      const double xi = dvec[static_cast<int64_t>(2 * pq + 1) * d.total + g];
+      re += h.x * xr - h.y * xi;
+      im += h.x * xi + h.y * xr;
     }
     int64_t j;
     double sg;
     if (set_source(pq / m, pq % m, sec, na, nb, a, b, d, &j, &sg)) {
-      const double2 x = gsrc[static_cast<int64_t>(pq) * gstride + j];
+      double2 x;
+      if (kPlanarG) {
+        x.x = gsrc[static_cast<int64_t>(2 * pq) * d.total + j];
+        x.y = gsrc[static_cast<int64_t>(2 * pq + 1) * d.total + j];
+      } else {
+        x.x = gsrc[2 * j];
+        x.y = gsrc[2 * j + 1];
+      }
       double2 h = make_double2(sg, 0.0);
       if (hg != nullptr) h = make_double2(sg * hg[pq].x, sg * hg[pq].y);
       re += h.x * x.x - h.y * x.y;
@@ -267,11 +277,13 @@ extern "C" int64_t fqehip_set_length(int norb, int nele) {
   return d.total;
 }
 
+// Workspace: D (2 m2 L doubles), Y (2 m2 L doubles), then the real block
+// form of V ((2 m2)^2 doubles).
 extern "C" int64_t fqehip_set_workspace_bytes(int norb, int nele) {
   SetDesc d;
   if (make_desc(norb, nele, &d) != FQEHIP_OK) return -1;
   const int64_t m2 = 4LL * norb * norb;
-  return 2 * m2 * d.total * static_cast<int64_t>(sizeof(double2));
+  return (4 * m2 * d.total + 4 * m2 * m2) * static_cast<int64_t>(sizeof(double));
 }
 
 extern "C" int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
@@ -286,9 +298,32 @@ extern "C" int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
   k_set_dvec<<<blocks_for(d.total, kSetBlock), kSetBlock, 0,
                as_stream(stream)>>>(d, m2,
                                     reinterpret_cast<const double2*>(d_coeff),
-                                    reinterpret_cast<double2*>(d_dvec));
+                                    d_dvec);
   FQEHIP_CHECK_LAUNCH("k_set_dvec");
   return FQEHIP_OK;
+}
+
+extern "C" int fqehip_set_contract(int norb, int nele, const double* d_v,
+                                   double* d_workspace, void* stream) {
+  clear_error();
+  SetDesc d;
+  if (int rc = make_desc(norb, nele, &d)) return rc;
+  FQEHIP_CHECK_ARG(d_v != nullptr && d_workspace != nullptr,
+                   "fqehip_set_contract: null pointer");
+  if (d.total == 0) return FQEHIP_OK;
+  const int m2 = 4 * norb * norb;
+  const int64_t plane = 2LL * m2 * d.total;
+  double* dv = d_workspace;
+  double* y = dv + plane;
+  double* a = y + plane;
+  const int64_t na = 4LL * m2 * m2;
+  k_set_expand_v<<<blocks_for(na, kSetBlock), kSetBlock, 0,
+                   as_stream(stream)>>>(m2,
+                                        reinterpret_cast<const double2*>(d_v),
+                                        a);
+  FQEHIP_CHECK_LAUNCH("k_set_expand_v");
+  return fqehip_pair_gemm(FQEHIP_C128, 2 * m2, 2 * m2, a, d.total, dv, y,
+                          stream);
 }
 
 extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
@@ -312,21 +347,18 @@ extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
   const double2* c = reinterpret_cast<const double2*>(d_coeff);
   double2* out = reinterpret_cast<double2*>(d_out);
   if (d_v == nullptr) {
-    k_set_out<<<nblk, kSetBlock, 0, s>>>(d, m2, nullptr, nullptr, h1, c, 0,
-                                         out);
+    k_set_out<false><<<nblk, kSetBlock, 0, s>>>(d, m2, nullptr, nullptr, h1,
+                                                d_coeff, out);
     FQEHIP_CHECK_LAUNCH("k_set_out");
     return FQEHIP_OK;
   }
-  FQEHIP_CHECK_ARG(m2 <= 65535, "fqehip_set_apply: too many pairs");
-  double2* dv = reinterpret_cast<double2*>(d_workspace);
-  double2* y = dv + static_cast<int64_t>(m2) * d.total;
+  double* dv = d_workspace;
+  double* y = dv + 2LL * m2 * d.total;
   k_set_dvec<<<nblk, kSetBlock, 0, s>>>(d, m2, c, dv);
   FQEHIP_CHECK_LAUNCH("k_set_dvec");
-  k_set_vd<<<dim3(nblk, m2), kSetBlock, 0, s>>>(
-      d.total, m2, reinterpret_cast<const double2*>(d_v), dv, y);
-  FQEHIP_CHECK_LAUNCH("k_set_vd");
-  k_set_out<<<nblk, kSetBlock, 0, s>>>(d, m2, h1, dv, nullptr, y, d.total,
-                                       out);
+  if (int rc = fqehip_set_contract(norb, nele, d_v, d_workspace, stream))
+    return rc;
+  k_set_out<true><<<nblk, kSetBlock, 0, s>>>(d, m2, h1, dv, nullptr, y, out);
   FQEHIP_CHECK_LAUNCH("k_set_out");
   return FQEHIP_OK;
 }

@@ -120,6 +120,8 @@ _SIGNATURES = {
     "fqehip_set_length": (c_int64, [c_int, c_int]),
     "fqehip_set_workspace_bytes": (c_int64, [c_int, c_int]),
     "fqehip_set_dvec": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "fqehip_set_contract": (c_int, [c_int, c_int, c_void_p, c_void_p,
+                                    c_void_p]),
     "fqehip_set_apply": (c_int, [c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_void_p, c_void_p, c_void_p]),
     "fqehip_mlp_features": (c_int, [c_int, c_int64, c_int64, c_void_p,

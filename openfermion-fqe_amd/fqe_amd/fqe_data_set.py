@@ -11,7 +11,8 @@ beta orbital p - norb.
 
 On the device the sectors are concatenated, ascending nbeta, into one
 complex128 vector; ``fqehip_set_apply`` / ``fqehip_set_dvec`` gather over
-it (no atomics, the same bits on every run).  Like the reference's
+it and contract on the FP64 matrix cores (no atomics, the same bits on
+every run).  Like the reference's
 ``_calculate_dvec_with_coeff``, which reads the neighbouring sectors of
 every sector, the set must hold every sector of its particle number;
 3- and 4-body tuples raise NotImplementedError.
@@ -95,12 +96,13 @@ class FqeDataSet:
             off += n
 
     def _dvec(self, vec: torch.Tensor) -> torch.Tensor:
+        """[(2 norb)^2, L] complex128 D[pq] = E_pq C."""
         m = 2 * self._norb
-        d = torch.empty((m * m, vec.numel()), dtype=torch.complex128,
+        d = torch.empty((m * m, 2, vec.numel()), dtype=torch.float64,
                         device=vec.device)
         _lib.call("fqehip_set_dvec", self._norb, self._nele, _lib.ptr(vec),
                   _lib.ptr(d), _lib.stream_ptr())
-        return d
+        return torch.complex(d[:, 0], d[:, 1]).contiguous()
 
     # -- apply -----------------------------------------------------------------
     def apply(self, array: Tuple[numpy.ndarray, ...]) -> 'FqeDataSet':
