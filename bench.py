@@ -522,10 +522,10 @@ def set_main(args):
                                 f"sector, {n} determinants, random Hermitian "
                                 f"spin-mixing h1/h2, seed 5")},
         "kernels_ms": {"k_set_dvec": round(t_dvec, 4),
-                       "contract (k_set_expand_v + k_gemm_f64)": round(t_vd, 4),
+                       "contract (k_set_expand_v + k_set_gemm)": round(t_vd, 4),
                        "k_set_out": round(t_apply - t_dvec - t_vd, 4),
                        "apply": round(t_apply, 4)},
-        "roofline": {"kernel": "k_gemm_f64 (set contraction)", "bound": "mfma",
+        "roofline": {"kernel": "k_set_gemm", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": 78.6,
                      "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4),
                      "traffic": None,

@@ -18,8 +18,7 @@
 // Kernels:
 //   k_set_dvec  D[pq][I] = sum_J <I|E_pq|J> C[J]   (one thread per I)
 //   k_set_expand_v  the real block form [[Vr, -Vi], [Vi, Vr]] of V
-//   k_gemm_f64  Y = V D as one real FP64 matrix-core GEMM, m = k = 2 M^2
-//               (fqehip_pair_gemm)
+//   k_set_gemm  Y = V D as one real FP64 matrix-core GEMM, m = k = 2 M^2
 //   k_set_out   out[I]   = sum_pq hd[pq] D[pq][I]
 //                        + sum_pq hg[pq] <I|E_pq|J> G[pq][J]
 // with (hd, D, hg, G) = (h1', D, 1, Y) for the 1+2-body apply and
@@ -34,6 +33,13 @@ namespace {
 constexpr int kSetMaxNorb = 16;
 constexpr int kSetMaxSec = kSetMaxNorb + 1;
 constexpr int kSetBlock = 256;
+// k_set_gemm tiles: a wave owns 16 x 16 tiles of kGemmRows row tiles by
+// kGemmCols column tiles, so each A value it loads feeds kGemmCols MFMAs and
+// each B value kGemmRows; a block of 4 waves covers 64 rows x 256 columns.
+constexpr int kGemmRows = 4;
+constexpr int kGemmCols = 4;
+
+typedef double set_double4 __attribute__((ext_vector_type(4)));
 
 struct SetDesc {
   int64_t total;                 // L
@@ -190,6 +196,63 @@ k_set_expand_v(int m2, const double2* __restrict__ v, double* __restrict__ a) {
   a[e] = (u == w) ? h.x : (u == 0 ? -h.y : h.y);
 }
 
+// y[m][ncol] = a[m][k] b[k][ncol], all real row-major FP64, on
+// v_mfma_f64_16x16x4f64 (A operand: row lane%16, k lane/16; B operand:
+// k lane/16, column lane%16; result item i: row lane/16 + 4 i, column
+// lane%16, the layout k_gemm_f64 stores).  Out-of-range rows, columns and
+// k are read as zero and not written.
+__global__ void __launch_bounds__(kSetBlock)
+k_set_gemm(int m, int k, const double* __restrict__ a, int64_t ncol,
+           const double* __restrict__ b, double* __restrict__ y) {
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t col0 =
+      (static_cast<int64_t>(blockIdx.x) * 4 + wave) * (16 * kGemmCols);
+  if (col0 >= ncol) return;
+  const int row0 = blockIdx.y * kGemmRows * 16;
+  const int kl = lane >> 4;
+  const int li = lane & 15;
+  set_double4 acc[kGemmRows][kGemmCols];
+#pragma unroll
+  for (int t = 0; t < kGemmRows; ++t)
+#pragma unroll
+    for (int u = 0; u < kGemmCols; ++u) acc[t][u] = set_double4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < k; k0 += 4) {
+    const int kk = k0 + kl;
+    const bool kok = kk < k;
+    double bv[kGemmCols];
+#pragma unroll
+    for (int u = 0; u < kGemmCols; ++u) {
+      const int64_t col = col0 + u * 16 + li;
+      bv[u] = (kok && col < ncol) ? b[static_cast<int64_t>(kk) * ncol + col]
+                                  : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < kGemmRows; ++t) {
+      const int r = row0 + t * 16 + li;
+      const double av =
+          (kok && r < m) ? a[static_cast<int64_t>(r) * k + kk] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kGemmCols; ++u)
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[u], acc[t][u],
+                                                         0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kGemmCols; ++u) {
+    const int64_t col = col0 + u * 16 + li;
+    if (col >= ncol) continue;
+#pragma unroll
+    for (int t = 0; t < kGemmRows; ++t) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = row0 + t * 16 + kl + 4 * i;
+        if (r < m) y[static_cast<int64_t>(r) * ncol + col] = acc[t][u][i];
+      }
+    }
+  }
+}
+
 // out[I] = sum_pq hd[pq] D[pq][I] + sum_pq hg[pq] sign G[pq][J].
 // hd / dvec may be null (no direct term); hg null means all ones.
 // G is the planar Y (kPlanarG) or the interleaved coefficients C.
@@ -322,8 +385,13 @@ extern "C" int fqehip_set_contract(int norb, int nele, const double* d_v,
                                         reinterpret_cast<const double2*>(d_v),
                                         a);
   FQEHIP_CHECK_LAUNCH("k_set_expand_v");
-  return fqehip_pair_gemm(FQEHIP_C128, 2 * m2, 2 * m2, a, d.total, dv, y,
-                          stream);
+  const int mk = 2 * m2;
+  dim3 grid(blocks_for(d.total, 4 * 16 * kGemmCols),
+            (mk + kGemmRows * 16 - 1) / (kGemmRows * 16));
+  k_set_gemm<<<grid, kSetBlock, 0, as_stream(stream)>>>(mk, mk, a, d.total,
+                                                        dv, y);
+  FQEHIP_CHECK_LAUNCH("k_set_gemm");
+  return FQEHIP_OK;
 }
 
 extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
