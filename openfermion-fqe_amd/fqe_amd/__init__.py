@@ -14,7 +14,8 @@ from . import settings
 from .fci_graph import FciGraph
 from .fqe_data import FqeData
 from .fqe_data_set import FqeDataSet
-from .hamiltonians import Hamiltonian, RestrictedHamiltonian
+from .hamiltonians import (Hamiltonian, RestrictedHamiltonian, General,
+                           GSOHamiltonian)
 from .util import vdot, alpha_beta_electrons
 from .wavefunction import Wavefunction as _Wavefunction
 from .wavefunction import build_hamiltonian
@@ -31,6 +32,16 @@ def get_wavefunction(nele: int, m_s: int, norb: int,
                      dtype=None) -> _Wavefunction:
     """reference ``src/fqe/_fqe_control.py:163-178``"""
     return _Wavefunction([[nele, m_s, norb]], dtype=dtype)
+
+
+def get_general_hamiltonian(tensors, e_0: complex = 0. + 0.j) -> General:
+    """reference ``src/fqe/_fqe_control.py:526-539``"""
+    return General(tensors, e_0=e_0)
+
+
+def get_gso_hamiltonian(tensors, e_0: complex = 0. + 0.j) -> GSOHamiltonian:
+    """reference ``src/fqe/_fqe_control.py:542-556``"""
+    return GSOHamiltonian(tensors, e_0=e_0)
 
 
 def get_restricted_hamiltonian(tensors, e_0: complex = 0. + 0.j

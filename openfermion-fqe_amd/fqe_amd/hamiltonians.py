@@ -121,3 +121,72 @@ class RestrictedHamiltonian(Hamiltonian):
 
     def transform(self, trans: numpy.ndarray) -> numpy.ndarray:
         return trans.conj().T @ self._tensor[2] @ trans
+
+
+class _SpinOrbitalHamiltonian(Hamiltonian):
+    """Tensors over 2 norb spin orbitals (p < norb alpha, p >= norb beta):
+    h1[p, q] a+_p a_q + h2[p, q, r, s] a+_p a+_q a_r a_s + ...  Shared body
+    of ``General`` and ``GSOHamiltonian`` (reference
+    ``src/fqe/hamiltonians/general_hamiltonian.py:23-120``,
+    ``src/fqe/hamiltonians/gso_hamiltonian.py:23-120``)."""
+
+    _label = "spin-orbital Hamiltonian"
+
+    def __init__(self, tensors: Tuple[numpy.ndarray, ...],
+                 e_0: complex = 0.0 + 0.0j) -> None:
+        super().__init__(e_0=e_0)
+        self._tensor: Dict[int, numpy.ndarray] = {}
+        for rank, matrix in enumerate(tensors):
+            if not isinstance(matrix, numpy.ndarray):
+                raise TypeError(
+                    "Arg tensors should be a tuple of numpy.ndarray, but "
+                    f"tensors[{rank}] = {type(tensors[rank])}.")
+            if matrix.ndim % 2:
+                raise ValueError("input tensor has an odd rank")
+            self._tensor[2 * (rank + 1)] = matrix
+        assert self._tensor, f"No matrix elements passed into the {self._label}."
+        self._quadratic = len(self._tensor) == 1 and 2 in self._tensor
+        self._dim = list(self._tensor.values())[0].shape[0]
+
+    def __eq__(self, other: object) -> bool:
+        if type(other) is not type(self):
+            return NotImplemented
+        if self.e_0() != other.e_0():
+            return False
+        if self._tensor.keys() != other._tensor.keys():
+            return False
+        return all(numpy.allclose(self._tensor[k], other._tensor[k])
+                   for k in self._tensor)
+
+    __hash__ = None
+
+    def dim(self) -> int:
+        return self._dim
+
+    def rank(self) -> int:
+        return 2 * len(self._tensor)
+
+    def tensor(self, rank: int) -> numpy.ndarray:
+        return self._tensor[rank]
+
+    def tensors(self) -> Tuple[numpy.ndarray, ...]:
+        return tuple(self._tensor[2 * (r + 1)] for r in range(len(self._tensor)))
+
+    def quadratic(self) -> bool:
+        return self._quadratic
+
+    def iht(self, time: float) -> Tuple[numpy.ndarray, ...]:
+        return tuple(-1.0j * time * self._tensor[2 * (r + 1)]
+                     for r in range(len(self._tensor)))
+
+
+class General(_SpinOrbitalHamiltonian):
+    """reference ``src/fqe/hamiltonians/general_hamiltonian.py:23``"""
+
+    _label = "GeneralHamiltonian"
+
+
+class GSOHamiltonian(_SpinOrbitalHamiltonian):
+    """reference ``src/fqe/hamiltonians/gso_hamiltonian.py:23``"""
+
+    _label = "GSOHamiltonian"

@@ -25,7 +25,9 @@ from . import _lib
 from . import engine
 from . import settings
 from .fqe_data import FqeData
-from .hamiltonians import Hamiltonian, RestrictedHamiltonian
+from .fqe_data_set import FqeDataSet
+from .hamiltonians import (Hamiltonian, RestrictedHamiltonian, General,
+                           GSOHamiltonian, _SpinOrbitalHamiltonian)
 from .util import alpha_beta_electrons, validate_tuple, vdot
 
 
@@ -39,11 +41,10 @@ def build_hamiltonian(ops, norb: int = 0, conserve_number: bool = True,
         validate_tuple(ops)
         if norb != 0 and ops[0].shape[0] == norb:
             out = RestrictedHamiltonian(ops, e_0=e_0)
-            out._conserve_number = conserve_number
-            return out
-        raise NotImplementedError(
-            "spin-orbital (General) Hamiltonians are outside the GPU "
-            "engine's path")
+        else:
+            out = General(ops, e_0=e_0)
+        out._conserve_number = conserve_number
+        return out
     raise TypeError('Expected FermionOperator'
                     ' but received {}.'.format(type(ops)))
 
@@ -166,9 +167,12 @@ class Wavefunction:
         hamil = build_hamiltonian(hamil, norb=self.norb(),
                                   conserve_number=self.conserve_number())
         self._check_number(hamil)
+        if isinstance(hamil, _SpinOrbitalHamiltonian):
+            return self._apply_spin_orbital(hamil)
         if not isinstance(hamil, RestrictedHamiltonian):
             raise NotImplementedError(
-                "the GPU engine applies RestrictedHamiltonian operators")
+                "the GPU engine applies restricted and spin-orbital "
+                "(General, GSOHamiltonian) operators")
         if hamil.dim() != self._norb:
             raise ValueError('Hamiltonian has incorrect size:'
                              ' expected {}'.format(self._norb) +
@@ -185,6 +189,46 @@ class Wavefunction:
         beta = e_0 if numpy.abs(e_0) > 1.e-15 else 0.0
         for sec in out._civec.values():
             sec.apply_inplace_scaled(array, alpha=1.0, beta=beta, op=op)
+        return out
+
+    def _number_sectors(self) -> Dict[int, FqeDataSet]:
+        """Sectors grouped by particle number into FqeDataSets (reference
+        ``src/fqe/wavefunction.py:442-471``); every spin sector of each
+        number must be present."""
+        norb = self.norb()
+        out = {}
+        for nele in sorted(set(key[0] for key in self._civec)):
+            maxalpha = min(norb, nele)
+            minalpha = nele - maxalpha
+            needed = {(nele, 2 * na - nele)
+                      for na in range(minalpha, maxalpha + 1)}
+            if set(self._civec.keys()).intersection(needed) != needed:
+                raise ValueError('Wave function space is not spin complete.')
+            sectors = {(na, nele - na): self._civec[(nele, 2 * na - nele)]
+                       for na in range(minalpha, maxalpha + 1)}
+            out[nele] = FqeDataSet(nele, norb, sectors)
+        return out
+
+    def _apply_spin_orbital(self, hamil) -> 'Wavefunction':
+        """H|psi> for spin-orbital tensors on a spin-broken wavefunction:
+        each particle number's FqeDataSet takes the apply, then e_0 psi is
+        added (reference ``src/fqe/wavefunction.py:349-357``, ``:366-399``).
+        Spin-conserving wavefunctions, whose sectors the reference applies
+        the spin-orbital tensors to one by one, are not supported here."""
+        expected = self._norb * 2
+        if hamil.dim() != expected:
+            raise ValueError('Hamiltonian has incorrect size:'
+                             ' expected {}'.format(expected) +
+                             ' provided {}'.format(hamil.dim()))
+        if self._conserve_spin:
+            raise NotImplementedError(
+                "spin-orbital operators are applied to spin-broken "
+                "wavefunctions (broken=['spin']) through FqeDataSet")
+        out = copy.deepcopy(self)
+        for nsector in out._number_sectors().values():
+            nsector.apply_inplace(hamil.tensors())
+        if numpy.abs(hamil.e_0()) > 1.e-15:
+            out.ax_plus_y(hamil.e_0(), self)
         return out
 
     def expectationValue(self, ops, brawfn: 'Wavefunction' = None) -> complex:
@@ -235,7 +279,12 @@ class Wavefunction:
             raise TypeError(
                 "expansion must be an int. You provided {}".format(expansion))
         assert algo in ['taylor', 'chebyshev']
-        if algo == 'chebyshev':
+        if isinstance(hamil, _SpinOrbitalHamiltonian):
+            if algo != 'taylor':
+                raise NotImplementedError(
+                    "spin-orbital operators take the Taylor propagator")
+            out = _taylor_spin_orbital(self, time, hamil, accuracy, expansion)
+        elif algo == 'chebyshev':
             from .chebyshev import chebyshev_evolve
             out = chebyshev_evolve(self, time, hamil, accuracy, expansion,
                                    spec_lim)
@@ -328,6 +377,27 @@ def _prepared(hamil: Optional[RestrictedHamiltonian], array,
         hamil._prepared[key] = engine.prepare_operator(hamil.tensors(), norb,
                                                        dev)
     return hamil._prepared[key]
+
+
+def _taylor_spin_orbital(wfn: Wavefunction, time: float, hamil,
+                         accuracy: float, expansion: int) -> Wavefunction:
+    """The reference's Taylor loop (``src/fqe/wavefunction.py:556-568``)
+    over spin-orbital applies: psi(t) = sum_k (-i t H)^k psi / k!, stopping
+    once ||(-i t H)^k psi|| / k! < accuracy, RuntimeError after
+    expansion - 1 terms."""
+    gen = General(hamil.iht(time))
+    gen._conserve_number = wfn.conserve_number()
+    result = copy.deepcopy(wfn)
+    work = copy.deepcopy(wfn)
+    for order in range(1, expansion):
+        work = work._apply_spin_orbital(gen)
+        coeff = 1.0 / math.factorial(order)
+        result.ax_plus_y(coeff, work)
+        if work.norm() * abs(coeff) < accuracy:
+            break
+    else:
+        raise RuntimeError("maximum taylor expansion limit reached")
+    return result
 
 
 def _taylor(wfn: Wavefunction, time: float, hamil: RestrictedHamiltonian,

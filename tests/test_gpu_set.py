@@ -1,9 +1,11 @@
-"""GPU parity of the spin-orbital set path (FqeDataSet apply and RDMs).
+"""GPU parity of the spin-orbital set path (FqeDataSet apply and RDMs, and
+General / GSOHamiltonian operators on spin-broken wavefunctions).
 
 Every test runs fqehip_set_apply / fqehip_set_dvec through the C ABI and
-checks it against the reference's set0202 fixtures and the spin-orbital
-oracle (pinned on the CPU by tests/test_set_oracle.py).  The path computes
-in complex128 with gathers only: results agree with the oracle to a
+checks it against the reference's set0202 fixtures, the spin-orbital oracle
+(pinned on the CPU by tests/test_set_oracle.py) or the exponential of the
+phased Jordan-Wigner matrix.  The path computes in complex128 (gathers and
+one FP64 matrix-core GEMM, no atomics): results agree with the oracle to a
 normwise relative error of 1e-12 and are bit-identical run to run.
 """
 
@@ -188,3 +190,81 @@ def test_set_errors(fqe):
     with pytest.raises(NotImplementedError):
         full.apply((numpy.eye(m), numpy.zeros((m,) * 4),
                     numpy.zeros((m,) * 6)))
+
+
+# --------------------------------------------------------------------------
+# Wavefunction-level routing: General / GSOHamiltonian on spin-broken
+# wavefunctions go through FqeDataSet per particle number (reference
+# src/fqe/wavefunction.py:315-364, :366-399, :442-471).
+# --------------------------------------------------------------------------
+
+def spin_broken_wfn(fqe, nele, norb, coeffs):
+    keys = sorted(coeffs)
+    wfn = fqe.Wavefunction([[nele, k[0] - k[1], norb] for k in keys],
+                           broken=['spin'])
+    for k in keys:
+        wfn.sector((nele, k[0] - k[1])).coeff = numpy.array(
+            coeffs[k], dtype=numpy.complex128)
+    return wfn
+
+
+def test_gso_apply_on_wavefunction_matches_fixture(fqe):
+    print_case("wavefunction gso apply")
+    z = numpy.load(GOLDEN)
+    norb = 2
+    keys = [(2, 0), (1, 1), (0, 2)]
+    coeffs = {k: z[f"c_{k[0]:02d}{k[1]:02d}{norb:02d}"] for k in keys}
+    wfn = spin_broken_wfn(fqe, 2, norb, coeffs)
+    e0 = 0.3 - 0.1j
+    out = fqe.apply(fqe.get_gso_hamiltonian((z["h1"], z["h2"]), e_0=e0), wfn)
+    for k in keys:
+        want = z[f"out12_{k[0]:02d}{k[1]:02d}{norb:02d}"] + e0 * coeffs[k]
+        got = out.sector((2, k[0] - k[1])).coeff
+        assert numpy.allclose(got, want, rtol=1e-12, atol=1e-12)
+    # a bare tuple of spin-orbital tensors builds a General Hamiltonian
+    out2 = wfn.apply((z["h1"], z["h2"]))
+    for k in keys:
+        assert numpy.allclose(out2.sector((2, k[0] - k[1])).coeff,
+                              z[f"out12_{k[0]:02d}{k[1]:02d}{norb:02d}"],
+                              rtol=1e-12, atol=1e-12)
+
+
+def test_gso_time_evolve_matches_dense_exponential(fqe):
+    print_case("wavefunction gso time_evolve")
+    import scipy.linalg
+    from oracle import jw_dense
+    nele, norb, t = 3, 3, 0.4
+    rng = numpy.random.default_rng(31)
+    sg = so.SetGraph(nele, norb)
+    coeffs = random_coeffs(sg, rng)
+    norm = numpy.sqrt(sum(numpy.linalg.norm(c) ** 2 for c in coeffs.values()))
+    coeffs = {k: c / norm for k, c in coeffs.items()}
+    h1, h2 = random_tensors(norb, rng, hermitian=True)
+    h1, h2 = 0.2 * h1, 0.05 * h2
+    wfn = spin_broken_wfn(fqe, nele, norb, coeffs)
+    out = fqe.time_evolve(wfn, t, fqe.get_gso_hamiltonian((h1, h2)))
+    ph = jw_dense.set_sector_phase(norb, nele)
+    mat = jw_dense.dense_set_operator(h1, h2, norb, nele)
+    mat = ph[:, None] * mat * ph[None, :]
+    vec = numpy.concatenate([coeffs[k].ravel() for k in sg.keys])
+    want = scipy.linalg.expm(-1j * t * mat) @ vec
+    got = numpy.concatenate([out.sector((nele, k[0] - k[1])).coeff.ravel()
+                             for k in sg.keys])
+    assert rel(got, want) < 1e-11
+
+
+def test_gso_routing_errors(fqe):
+    print_case("wavefunction gso errors")
+    norb = 2
+    m = 2 * norb
+    ham = fqe.get_gso_hamiltonian((numpy.eye(m),))
+    partial = fqe.Wavefunction([[2, 2, norb], [2, 0, norb]], broken=['spin'])
+    with pytest.raises(ValueError, match="spin complete"):
+        fqe.apply(ham, partial)
+    conserving = fqe.Wavefunction([[2, 0, norb]])
+    with pytest.raises(NotImplementedError):
+        fqe.apply(ham, conserving)
+    full = fqe.Wavefunction([[2, 2, norb], [2, 0, norb], [2, -2, norb]],
+                            broken=['spin'])
+    with pytest.raises(ValueError, match="incorrect size"):
+        fqe.apply(fqe.get_gso_hamiltonian((numpy.eye(3),)), full)
