@@ -201,7 +201,7 @@ k_set_expand_v(int m2, const double2* __restrict__ v, double* __restrict__ a) {
 // k lane/16, column lane%16; result item i: row lane/16 + 4 i, column
 // lane%16, the layout k_gemm_f64 stores).  Out-of-range rows, columns and
 // k are read as zero and not written.
-__global__ void __launch_bounds__(kSetBlock)
+__global__ void __launch_bounds__(kSetBlock, 2)
 k_set_gemm(int m, int k, const double* __restrict__ a, int64_t ncol,
            const double* __restrict__ b, double* __restrict__ y) {
   const int wave = threadIdx.x >> 6;
@@ -217,26 +217,38 @@ k_set_gemm(int m, int k, const double* __restrict__ a, int64_t ncol,
   for (int t = 0; t < kGemmRows; ++t)
 #pragma unroll
     for (int u = 0; u < kGemmCols; ++u) acc[t][u] = set_double4{0, 0, 0, 0};
-  for (int k0 = 0; k0 < k; k0 += 4) {
+  // Operands of k step k0 are loaded one step ahead, so the loads of the
+  // next step are in flight while this step's MFMAs run.
+  double av[kGemmRows], bv[kGemmCols];
+  auto load = [&](int k0, double* an, double* bn) {
     const int kk = k0 + kl;
     const bool kok = kk < k;
-    double bv[kGemmCols];
 #pragma unroll
     for (int u = 0; u < kGemmCols; ++u) {
       const int64_t col = col0 + u * 16 + li;
-      bv[u] = (kok && col < ncol) ? b[static_cast<int64_t>(kk) * ncol + col]
+      bn[u] = (kok && col < ncol) ? b[static_cast<int64_t>(kk) * ncol + col]
                                   : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < kGemmRows; ++t) {
       const int r = row0 + t * 16 + li;
-      const double av =
-          (kok && r < m) ? a[static_cast<int64_t>(r) * k + kk] : 0.0;
+      an[t] = (kok && r < m) ? a[static_cast<int64_t>(r) * k + kk] : 0.0;
+    }
+  };
+  load(0, av, bv);
+  for (int k0 = 0; k0 < k; k0 += 4) {
+    double an[kGemmRows], bn[kGemmCols];
+    load(k0 + 4, an, bn);   // zeros past the end of k
+#pragma unroll
+    for (int t = 0; t < kGemmRows; ++t)
 #pragma unroll
       for (int u = 0; u < kGemmCols; ++u)
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[u], acc[t][u],
-                                                         0, 0, 0);
-    }
+        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv[u],
+                                                         acc[t][u], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < kGemmRows; ++t) av[t] = an[t];
+#pragma unroll
+    for (int u = 0; u < kGemmCols; ++u) bv[u] = bn[u];
   }
 #pragma unroll
   for (int u = 0; u < kGemmCols; ++u) {
