@@ -383,6 +383,19 @@ def kernel_work(op, graph, n, esz, mixed):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "apply_norb14_pmc_traffic.md")
+SET_TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                                "set_apply_norb10_pmc_traffic.md")
+
+
+def _traffic_block(path):
+    if not os.path.exists(path):
+        return {}
+    text = open(path).read()
+    i = text.find("```json")
+    if i < 0:
+        return {}
+    j = text.find("```", i + 7)
+    return json.loads(text[i + 7:j])["traffic_bytes_per_launch"]
 
 
 def measured_traffic(norb, na, nb):
@@ -470,6 +483,13 @@ def set_main(args):
     v = -numpy.moveaxis(h2, 1, 2)
     h1p = h1 - numpy.einsum("ikkj->ij", v)
     dev = L.device()
+    if args.copy_probe:
+        # calibration copy for the FETCH_SIZE / WRITE_SIZE counter passes
+        src = torch.empty(args.copy_probe, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     tc = torch.from_numpy(coeff).to(dev)
     th1 = torch.from_numpy(numpy.ascontiguousarray(h1p)).to(dev)
     tv = torch.from_numpy(numpy.ascontiguousarray(v.reshape(m2, m2))).to(dev)
@@ -528,7 +548,8 @@ def set_main(args):
         "roofline": {"kernel": "k_set_gemm", "bound": "mfma",
                      "achieved": round(achieved, 3), "peak": 78.6,
                      "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4),
-                     "traffic": None,
+                     "traffic": (_traffic_block(SET_TRAFFIC_FILE).get(
+                         "k_set_gemm") if norb == 10 else None),
                      "note": ("8 M^4 L algorithmic FP64 flops per "
                               "contraction (real block GEMM, m = k = 2 M^2, "
                               "L columns); time includes the V expansion")},
