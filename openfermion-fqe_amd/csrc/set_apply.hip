@@ -38,6 +38,8 @@ constexpr int kSetBlock = 256;
 // each B value kGemmRows; a block of 4 waves covers 64 rows x 256 columns.
 constexpr int kGemmRows = 4;
 constexpr int kGemmCols = 4;
+constexpr int kGemmKc = 32;   // k values of a staged in LDS per chunk
+static_assert(kGemmRows * 16 * kGemmKc % 256 == 0, "even LDS fill");
 
 typedef double set_double4 __attribute__((ext_vector_type(4)));
 
@@ -204,11 +206,16 @@ k_set_expand_v(int m2, const double2* __restrict__ v, double* __restrict__ a) {
 __global__ void __launch_bounds__(kSetBlock, 2)
 k_set_gemm(int m, int k, const double* __restrict__ a, int64_t ncol,
            const double* __restrict__ b, double* __restrict__ y) {
+  // The block's 4 waves share its kGemmRows*16 rows of a: each chunk of
+  // kGemmKc k values of those rows is staged in LDS once, with every
+  // thread of the block taking part.  The b operands of k step k0 + 4 are
+  // loaded while the MFMAs of step k0 run.  No wave leaves early, so every
+  // wave reaches every barrier.
+  __shared__ double sa[kGemmRows * 16][kGemmKc + 1];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t col0 =
       (static_cast<int64_t>(blockIdx.x) * 4 + wave) * (16 * kGemmCols);
-  if (col0 >= ncol) return;
   const int row0 = blockIdx.y * kGemmRows * 16;
   const int kl = lane >> 4;
   const int li = lane & 15;
@@ -217,38 +224,43 @@ k_set_gemm(int m, int k, const double* __restrict__ a, int64_t ncol,
   for (int t = 0; t < kGemmRows; ++t)
 #pragma unroll
     for (int u = 0; u < kGemmCols; ++u) acc[t][u] = set_double4{0, 0, 0, 0};
-  // Operands of k step k0 are loaded one step ahead, so the loads of the
-  // next step are in flight while this step's MFMAs run.
-  double av[kGemmRows], bv[kGemmCols];
-  auto load = [&](int k0, double* an, double* bn) {
+  auto load_b = [&](int k0, double* bn) {
     const int kk = k0 + kl;
-    const bool kok = kk < k;
 #pragma unroll
     for (int u = 0; u < kGemmCols; ++u) {
       const int64_t col = col0 + u * 16 + li;
-      bn[u] = (kok && col < ncol) ? b[static_cast<int64_t>(kk) * ncol + col]
-                                  : 0.0;
-    }
-#pragma unroll
-    for (int t = 0; t < kGemmRows; ++t) {
-      const int r = row0 + t * 16 + li;
-      an[t] = (kok && r < m) ? a[static_cast<int64_t>(r) * k + kk] : 0.0;
+      bn[u] = (kk < k && col < ncol) ? b[static_cast<int64_t>(kk) * ncol + col]
+                                     : 0.0;
     }
   };
-  load(0, av, bv);
-  for (int k0 = 0; k0 < k; k0 += 4) {
-    double an[kGemmRows], bn[kGemmCols];
-    load(k0 + 4, an, bn);   // zeros past the end of k
+  double bv[kGemmCols];
+  load_b(0, bv);
+  for (int kc0 = 0; kc0 < k; kc0 += kGemmKc) {
+    __syncthreads();   // the previous chunk's reads of sa are done
 #pragma unroll
-    for (int t = 0; t < kGemmRows; ++t)
+    for (int j = 0; j < kGemmRows * 16 * kGemmKc / kSetBlock; ++j) {
+      const int e = threadIdx.x + j * kSetBlock;
+      const int r = e / kGemmKc, c = e % kGemmKc;
+      const int gr = row0 + r, gc = kc0 + c;
+      sa[r][c] = (gr < m && gc < k) ? a[static_cast<int64_t>(gr) * k + gc]
+                                    : 0.0;
+    }
+    __syncthreads();
 #pragma unroll
-      for (int u = 0; u < kGemmCols; ++u)
-        acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[t], bv[u],
-                                                         acc[t][u], 0, 0, 0);
+    for (int ks = 0; ks < kGemmKc; ks += 4) {
+      double bn[kGemmCols];
+      load_b(kc0 + ks + 4, bn);   // zeros past the end of k
 #pragma unroll
-    for (int t = 0; t < kGemmRows; ++t) av[t] = an[t];
+      for (int t = 0; t < kGemmRows; ++t) {
+        const double av = sa[t * 16 + li][ks + kl];
 #pragma unroll
-    for (int u = 0; u < kGemmCols; ++u) bv[u] = bn[u];
+        for (int u = 0; u < kGemmCols; ++u)
+          acc[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[u],
+                                                           acc[t][u], 0, 0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < kGemmCols; ++u) bv[u] = bn[u];
+    }
   }
 #pragma unroll
   for (int u = 0; u < kGemmCols; ++u) {
