@@ -17,6 +17,12 @@ with alpha orbitals on modes 0..norb-1 and beta orbitals on modes
 norb..2*norb-1, creation operators ordered by ascending mode.
 
 Addresses are the lexicographic ranks of the ascending occupation tuples.
+
+The second half builds general spin-orbital operators (modes = spin-orbital
+indices, spin-flip terms included) over every sector of one particle number
+(``dense_set_operator``, ``set_expectation``); ``set_sector_phase`` gives
+the phases that turn these determinants into the reference's FqeDataSet
+ones.
 """
 
 from __future__ import annotations
