@@ -17,12 +17,13 @@
 //
 // Kernels:
 //   k_set_dvec  D[pq][I] = sum_J <I|E_pq|J> C[J]   (one thread per I)
-//   k_set_expand_v  the real block form [[Vr, -Vi], [Vi, Vr]] of V
-//   k_set_gemm  Y = V D as one real FP64 matrix-core GEMM, m = k = 2 M^2
-//   k_set_out   out[I]   = sum_pq hd[pq] D[pq][I]
-//                        + sum_pq hg[pq] <I|E_pq|J> G[pq][J]
-// with (hd, D, hg, G) = (h1', D, 1, Y) for the 1+2-body apply and
-// (-, -, h1, C) for the 1-body apply.  D and Y are planar per pair:
+//   k_set_expand_v  the real block form [[Vr, -Vi], [Vi, Vr]] of V, with
+//               the block form of the row h1' under it for the apply
+//   k_set_gemm  Y = V D as one real FP64 matrix-core GEMM, k = 2 M^2 and
+//               m = 2 M^2 (+ 2 rows that hold sum_pq h1'[pq] D[pq])
+//   k_set_out   out[I]   = Y[h1' row][I] + sum_pq <I|E_pq|J> Y[pq][J]
+//               for the 1+2-body apply, or
+//               out[I]   = sum_pq h1[pq] <I|E_pq|J> C[J] for the 1-body one.  D and Y are planar per pair:
 // row 2 pq holds the real parts, row 2 pq + 1 the imaginary parts, L
 // doubles each, so the GEMM sees them as real 2 M^2 x L matrices.
 #include "fqehip_common.h"
@@ -186,14 +187,17 @@ k_set_dvec(SetDesc d, int m2, const double2* __restrict__ c,
 }
 
 // a[(2 pq + u) * 2 m2 + 2 rs + w] = [[Vr, -Vi], [Vi, Vr]][u][w] of
-// V[pq][rs] (complex128 [m2][m2]); one thread per element of a.
+// V[pq][rs] (complex128 [m2][m2]) for rows < 2 m2; rows 2 m2 and 2 m2 + 1
+// (when rows = 2 m2 + 2) hold the same block form of the row h1[rs].  One
+// thread per element of a.
 __global__ void __launch_bounds__(kSetBlock)
-k_set_expand_v(int m2, const double2* __restrict__ v, double* __restrict__ a) {
+k_set_expand_v(int m2, int rows, const double2* __restrict__ v,
+               const double2* __restrict__ h1, double* __restrict__ a) {
   const int64_t n2 = 2LL * m2;
   const int64_t e = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
-  if (e >= n2 * n2) return;
+  if (e >= rows * n2) return;
   const int64_t row = e / n2, col = e % n2;
-  const double2 h = v[(row >> 1) * m2 + (col >> 1)];
+  const double2 h = row < n2 ? v[(row >> 1) * m2 + (col >> 1)] : h1[col >> 1];
   const int u = static_cast<int>(row & 1), w = static_cast<int>(col & 1);
   a[e] = (u == w) ? h.x : (u == 0 ? -h.y : h.y);
 }
@@ -277,13 +281,12 @@ k_set_gemm(int m, int k, const double* __restrict__ a, int64_t ncol,
   }
 }
 
-// out[I] = sum_pq hd[pq] D[pq][I] + sum_pq hg[pq] sign G[pq][J].
-// hd / dvec may be null (no direct term); hg null means all ones.
-// G is the planar Y (kPlanarG) or the interleaved coefficients C.
+// kPlanarG: out[I] = Y[2 m2][I] + i Y[2 m2 + 1][I]
+//                    + sum_pq <I|E_pq|J> (Y[2 pq][J] + i Y[2 pq + 1][J]);
+// otherwise: out[I] = sum_pq hg[pq] <I|E_pq|J> C[J], C interleaved.
 template <bool kPlanarG>
 __global__ void __launch_bounds__(kSetBlock)
-k_set_out(SetDesc d, int m2, const double2* __restrict__ hd,
-          const double* __restrict__ dvec, const double2* __restrict__ hg,
+k_set_out(SetDesc d, int m2, const double2* __restrict__ hg,
           const double* __restrict__ gsrc, double2* __restrict__ out) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * kSetBlock + threadIdx.x;
   if (g >= d.total) return;
@@ -292,29 +295,23 @@ k_set_out(SetDesc d, int m2, const double2* __restrict__ hd,
   locate(g, d, &sec, &na, &nb, &a, &b);
   const int m = 2 * d.norb;
   double re = 0.0, im = 0.0;
+  if (kPlanarG) {
+    re = gsrc[static_cast<int64_t>(2 * m2) * d.total + g];
+    im = gsrc[static_cast<int64_t>(2 * m2 + 1) * d.total + g];
+  }
   for (int pq = 0; pq < m2; ++pq) {
-    if (dvec != nullptr) {
-      const double2 h = hd[pq];
-      const double xr = dvec[static_cast<int64_t>(2 * pq) * d.total + g];
-      const double xi = dvec[static_cast<int64_t>(2 * pq + 1) * d.total + g];
-      re += h.x * xr - h.y * xi;
-      im += h.x * xi + h.y * xr;
-    }
     int64_t j;
     double sg;
     if (set_source(pq / m, pq % m, sec, na, nb, a, b, d, &j, &sg)) {
-      double2 x;
       if (kPlanarG) {
-        x.x = gsrc[static_cast<int64_t>(2 * pq) * d.total + j];
-        x.y = gsrc[static_cast<int64_t>(2 * pq + 1) * d.total + j];
+        re += sg * gsrc[static_cast<int64_t>(2 * pq) * d.total + j];
+        im += sg * gsrc[static_cast<int64_t>(2 * pq + 1) * d.total + j];
       } else {
-        x.x = gsrc[2 * j];
-        x.y = gsrc[2 * j + 1];
+        const double xr = gsrc[2 * j], xi = gsrc[2 * j + 1];
+        const double hr = sg * hg[pq].x, hi = sg * hg[pq].y;
+        re += hr * xr - hi * xi;
+        im += hr * xi + hi * xr;
       }
-      double2 h = make_double2(sg, 0.0);
-      if (hg != nullptr) h = make_double2(sg * hg[pq].x, sg * hg[pq].y);
-      re += h.x * x.x - h.y * x.y;
-      im += h.x * x.y + h.y * x.x;
     }
   }
   out[g] = make_double2(re, im);
@@ -364,13 +361,14 @@ extern "C" int64_t fqehip_set_length(int norb, int nele) {
   return d.total;
 }
 
-// Workspace: D (2 m2 L doubles), Y (2 m2 L doubles), then the real block
-// form of V ((2 m2)^2 doubles).
+// Workspace: D (2 m2 L doubles), Y ((2 m2 + 2) L doubles), then the real
+// block form of V with the h1' row ((2 m2 + 2) 2 m2 doubles).
 extern "C" int64_t fqehip_set_workspace_bytes(int norb, int nele) {
   SetDesc d;
   if (make_desc(norb, nele, &d) != FQEHIP_OK) return -1;
   const int64_t m2 = 4LL * norb * norb;
-  return (4 * m2 * d.total + 4 * m2 * m2) * static_cast<int64_t>(sizeof(double));
+  return ((4 * m2 + 2) * d.total + (2 * m2 + 2) * 2 * m2) *
+         static_cast<int64_t>(sizeof(double));
 }
 
 extern "C" int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
@@ -390,6 +388,32 @@ extern "C" int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
   return FQEHIP_OK;
 }
 
+namespace fqehip {
+namespace {
+
+// Y = [V; h1'] D on the workspace (h1 may be null: V rows only).
+int set_contract(const SetDesc& d, int m2, const double* v, const double* h1,
+                 double* ws, hipStream_t st) {
+  const int n2 = 2 * m2;
+  const int rows = n2 + (h1 != nullptr ? 2 : 0);
+  double* dv = ws;
+  double* y = dv + static_cast<int64_t>(n2) * d.total;
+  double* a = y + static_cast<int64_t>(n2 + 2) * d.total;
+  const int64_t na = static_cast<int64_t>(rows) * n2;
+  k_set_expand_v<<<blocks_for(na, kSetBlock), kSetBlock, 0, st>>>(
+      m2, rows, reinterpret_cast<const double2*>(v),
+      reinterpret_cast<const double2*>(h1), a);
+  FQEHIP_CHECK_LAUNCH("k_set_expand_v");
+  dim3 grid(blocks_for(d.total, 4 * 16 * kGemmCols),
+            (rows + kGemmRows * 16 - 1) / (kGemmRows * 16));
+  k_set_gemm<<<grid, kSetBlock, 0, st>>>(rows, n2, a, d.total, dv, y);
+  FQEHIP_CHECK_LAUNCH("k_set_gemm");
+  return FQEHIP_OK;
+}
+
+}  // namespace
+}  // namespace fqehip
+
 extern "C" int fqehip_set_contract(int norb, int nele, const double* d_v,
                                    double* d_workspace, void* stream) {
   clear_error();
@@ -398,24 +422,8 @@ extern "C" int fqehip_set_contract(int norb, int nele, const double* d_v,
   FQEHIP_CHECK_ARG(d_v != nullptr && d_workspace != nullptr,
                    "fqehip_set_contract: null pointer");
   if (d.total == 0) return FQEHIP_OK;
-  const int m2 = 4 * norb * norb;
-  const int64_t plane = 2LL * m2 * d.total;
-  double* dv = d_workspace;
-  double* y = dv + plane;
-  double* a = y + plane;
-  const int64_t na = 4LL * m2 * m2;
-  k_set_expand_v<<<blocks_for(na, kSetBlock), kSetBlock, 0,
-                   as_stream(stream)>>>(m2,
-                                        reinterpret_cast<const double2*>(d_v),
-                                        a);
-  FQEHIP_CHECK_LAUNCH("k_set_expand_v");
-  const int mk = 2 * m2;
-  dim3 grid(blocks_for(d.total, 4 * 16 * kGemmCols),
-            (mk + kGemmRows * 16 - 1) / (kGemmRows * 16));
-  k_set_gemm<<<grid, kSetBlock, 0, as_stream(stream)>>>(mk, mk, a, d.total,
-                                                        dv, y);
-  FQEHIP_CHECK_LAUNCH("k_set_gemm");
-  return FQEHIP_OK;
+  return set_contract(d, 4 * norb * norb, d_v, nullptr, d_workspace,
+                      as_stream(stream));
 }
 
 extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
@@ -439,8 +447,7 @@ extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
   const double2* c = reinterpret_cast<const double2*>(d_coeff);
   double2* out = reinterpret_cast<double2*>(d_out);
   if (d_v == nullptr) {
-    k_set_out<false><<<nblk, kSetBlock, 0, s>>>(d, m2, nullptr, nullptr, h1,
-                                                d_coeff, out);
+    k_set_out<false><<<nblk, kSetBlock, 0, s>>>(d, m2, h1, d_coeff, out);
     FQEHIP_CHECK_LAUNCH("k_set_out");
     return FQEHIP_OK;
   }
@@ -448,9 +455,8 @@ extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
   double* y = dv + 2LL * m2 * d.total;
   k_set_dvec<<<nblk, kSetBlock, 0, s>>>(d, m2, c, dv);
   FQEHIP_CHECK_LAUNCH("k_set_dvec");
-  if (int rc = fqehip_set_contract(norb, nele, d_v, d_workspace, stream))
-    return rc;
-  k_set_out<true><<<nblk, kSetBlock, 0, s>>>(d, m2, h1, dv, nullptr, y, out);
+  if (int rc = set_contract(d, m2, d_v, d_h1, d_workspace, s)) return rc;
+  k_set_out<true><<<nblk, kSetBlock, 0, s>>>(d, m2, nullptr, y, out);
   FQEHIP_CHECK_LAUNCH("k_set_out");
   return FQEHIP_OK;
 }

@@ -73,7 +73,7 @@ def test_set_sizes_and_argument_errors_without_gpu_work():
             assert n == math.comb(2 * norb, nele)
             m2 = (2 * norb) ** 2
             assert lib.fqehip_set_workspace_bytes(norb, nele) == (
-                (4 * m2 * n + 4 * m2 * m2) * 8)
+                ((4 * m2 + 2) * n + (2 * m2 + 2) * 2 * m2) * 8)
     assert lib.fqehip_set_length(17, 2) == -1
     assert lib.fqehip_set_length(3, 7) == -1
     rc = lib.fqehip_set_apply(17, 2, None, None, None, None, None, None)
