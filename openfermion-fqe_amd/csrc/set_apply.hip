@@ -34,6 +34,7 @@ namespace {
 constexpr int kSetMaxNorb = 16;
 constexpr int kSetMaxSec = kSetMaxNorb + 1;
 constexpr int kSetBlock = 256;
+constexpr int kSetPairChunk = 64;   // pairs per k_set_dvec block row
 // k_set_gemm tiles: a wave owns 16 x 16 tiles of kGemmRows row tiles by
 // kGemmCols column tiles, so each A value it loads feeds kGemmCols MFMAs and
 // each B value kGemmRows; a block of 4 waves covers 64 rows x 256 columns.
@@ -163,6 +164,8 @@ __device__ __forceinline__ void locate(int64_t g, const SetDesc& d, int* sec,
   *b = lex_unrank(static_cast<int32_t>(r % lenb), d.norb, *nb, d);
 }
 
+// One thread per determinant and blockIdx.y-th chunk of kSetPairChunk
+// pairs, so that the grid has enough blocks to fill the chip.
 __global__ void __launch_bounds__(kSetBlock)
 k_set_dvec(SetDesc d, int m2, const double2* __restrict__ c,
            double* __restrict__ dvec) {
@@ -172,7 +175,9 @@ k_set_dvec(SetDesc d, int m2, const double2* __restrict__ c,
   uint32_t a, b;
   locate(g, d, &sec, &na, &nb, &a, &b);
   const int m = 2 * d.norb;
-  for (int pq = 0; pq < m2; ++pq) {
+  const int pq0 = blockIdx.y * kSetPairChunk;
+  const int pq1 = pq0 + kSetPairChunk < m2 ? pq0 + kSetPairChunk : m2;
+  for (int pq = pq0; pq < pq1; ++pq) {
     int64_t j;
     double sg;
     double re = 0.0, im = 0.0;
@@ -380,8 +385,9 @@ extern "C" int fqehip_set_dvec(int norb, int nele, const double* d_coeff,
                    "fqehip_set_dvec: null pointer");
   if (d.total == 0) return FQEHIP_OK;
   const int m2 = 4 * norb * norb;
-  k_set_dvec<<<blocks_for(d.total, kSetBlock), kSetBlock, 0,
-               as_stream(stream)>>>(d, m2,
+  const dim3 grid(blocks_for(d.total, kSetBlock),
+                  (m2 + kSetPairChunk - 1) / kSetPairChunk);
+  k_set_dvec<<<grid, kSetBlock, 0, as_stream(stream)>>>(d, m2,
                                     reinterpret_cast<const double2*>(d_coeff),
                                     d_dvec);
   FQEHIP_CHECK_LAUNCH("k_set_dvec");
@@ -453,7 +459,8 @@ extern "C" int fqehip_set_apply(int norb, int nele, const double* d_h1,
   }
   double* dv = d_workspace;
   double* y = dv + 2LL * m2 * d.total;
-  k_set_dvec<<<nblk, kSetBlock, 0, s>>>(d, m2, c, dv);
+  const dim3 dgrid(nblk, (m2 + kSetPairChunk - 1) / kSetPairChunk);
+  k_set_dvec<<<dgrid, kSetBlock, 0, s>>>(d, m2, c, dv);
   FQEHIP_CHECK_LAUNCH("k_set_dvec");
   if (int rc = set_contract(d, m2, d_v, d_h1, d_workspace, s)) return rc;
   k_set_out<true><<<nblk, kSetBlock, 0, s>>>(d, m2, nullptr, y, out);
