@@ -289,6 +289,26 @@ int fqehip_apply(int precision, const fqehip_sector* s,
                  double* d_acc, double* d_norm_partials,
                  const int32_t* d_done, void* d_workspace, void* stream);
 
+/* fqehip_apply without a caller workspace and without any device
+ * allocation.  No counterpart in the reference (its C kernels allocate
+ * scratch with malloc, src/fqe/lib/fqe_data.c:879-996): it is for callers
+ * that cannot allocate device memory, such as code under HIP graph capture
+ * and tight serving loops.  The workspace is a library-owned device array
+ * of FQEHIP_NOALLOC_SCRATCH_BYTES per GPU, held from library load until
+ * the process ends; a sector whose apply needs more returns
+ * FQEHIP_ERR_UNSUPPORTED (fqehip_apply_workspace_bytes bounds the need;
+ * the fused path needs (nslot+1)*2*lena*lenb elements, e.g. norb=12
+ * 6a6b complex64 fits, complex128 does not).  Arguments and results are
+ * those of fqehip_apply.  All calls on one GPU share its array: calls that
+ * overlap on one GPU (other streams, other host threads) race. */
+#define FQEHIP_NOALLOC_SCRATCH_BYTES (256ll * 1024 * 1024)
+int fqehip_apply_noalloc(int precision, const fqehip_sector* s,
+                         const fqehip_operator* op, const void* d_coeff,
+                         double alpha_re, double alpha_im, double beta_re,
+                         double beta_im, const void* d_cin, void* d_out,
+                         double* d_acc, double* d_norm_partials,
+                         const int32_t* d_done, void* stream);
+
 /* Taylor series psi(t) = sum_k (-i t H)^k psi / k! for a real operator
  * (time folded into the scalar), reference
  * src/fqe/wavefunction.py:556-568.  d_term (storage precision) holds psi

@@ -155,6 +155,11 @@ _SIGNATURES = {
                              c_void_p, c_double, c_double, c_double, c_double,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),
+    "fqehip_apply_noalloc": (c_int, [c_int, POINTER(Sector),
+                                     POINTER(Operator), c_void_p, c_double,
+                                     c_double, c_double, c_double, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
     "fqehip_taylor": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
                               c_double, c_double, c_int, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p,
