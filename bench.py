@@ -4,25 +4,42 @@ Workload (the headline configuration): norb=14, 7 alpha + 7 beta electrons
 (3,432 x 3,432 = 11,778,624 determinants), complex coefficients with real
 and imaginary parts i.i.d. N(0,1) from seed 2, normalised; h1/h2 from 14
 Cholesky-like factors (fqe_amd.synthetic.workload).  A step is
-one apply H|C>.
+one apply H|C> with C resident in HBM.
 
-The engine runs it twice: FP64 (complex128 storage, FP64 MFMA GEMM) and
-mixed precision (complex64 storage, BF16 hi/lo split MFMA GEMM with FP32
-accumulation).  ``value`` is the mixed engine's applies/s; the FP64 rate,
-the speed-up and the relative errors (mixed vs FP64 on the GPU, both vs
-the reference's complex128 apply) are reported beside it.
+The engine runs it three ways:
+  * mixed precision at the metric's 1e-9 gate (``value``): complex128
+    storage, both contractions as exact products of 8-bit digits on the
+    int8 matrix cores (Ozaki scheme, int32 accumulation, FP64 assembly);
+  * FP64 (complex128 storage, FP64 matrix cores) beside it (``fp64``);
+  * the BF16x3 engine (complex64 storage, BF16 hi/lo products with FP32
+    accumulation), labelled separately (``bf16x3_c64``): faster, but about
+    4e-6 from the FP64 result, outside the gate.
+Each result states its relative error against the FP64 GPU result and the
+reference's own complex128 apply (its compiled C lm kernels, oracle/_ref).
 
 Per-kernel durations are measured with HIP events on the stream the
 kernels run on; ``roofline`` is given for the dominant kernel of the
-mixed engine.  ``cpu_baseline`` times the reference's own compiled C
-(lm kernels, complex128, oracle/_ref) on the host cores for one apply.
+headline engine.  ``cpu_baseline`` times the reference's compiled C on the
+host cores: one warm-up, then the median of five applies, all cores,
+OMP_PROC_BIND=close, plus a single-thread record on the norb=12 sector.
+``cfg1`` is the production-size plain-method workload (norb=12 6a6b, seed
+1): 100 chained applies plus time_evolve(0.05), FP64 on the GPU against
+the reference's C on the host.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
 
+import os
+
+# host cores available to this process, counted before any OpenMP runtime
+# binds the main thread
+HOST_CORES = len(os.sched_getaffinity(0))
+# the CPU baseline's OpenMP threads stay on neighbouring cores; set before
+# any OpenMP runtime is loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
+
 import argparse
 import json
-import os
 import sys
 import time
 
@@ -34,7 +51,7 @@ for _p in (ROOT, os.path.join(ROOT, "openfermion-fqe_amd")):
 METRIC = ("applies/sec (norb=14,7α7β): mixed vs FP64 on GPU at "
           "≤1e-9 rel err")
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"f64": 78.6, "bf16": 2500.0}
+PEAK_TFLOPS = {"f64": 78.6, "bf16": 2500.0, "i8": 5000.0}
 
 
 def parse():
@@ -47,8 +64,11 @@ def parse():
     ap.add_argument("--nbeta", type=int, default=7)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--modes", default="fp64,mixed",
-                    help="engines to run (profiling runs use one)")
+    ap.add_argument("--modes", default="fp64,ozaki,bf16x3",
+                    help="engines to run (profiling runs use one): fp64, "
+                         "ozaki (the headline), bf16x3")
+    ap.add_argument("--no-cfg1", action="store_true",
+                    help="skip the norb=12 chained-apply workload")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first copy this many bytes device-to-device (the "
                          "calibration kernel of HBM traffic counters)")
@@ -240,7 +260,8 @@ class EngineRun:
     """Device buffers and the kernel launches of one apply (fused path when
     the engine supports the sector, else gather / GEMM / scatter)."""
 
-    def __init__(self, graph, op, coeff_np, torch_dtype, fused=True):
+    def __init__(self, graph, op, coeff_np, torch_dtype, fused=True,
+                 contraction="fp64"):
         import ctypes
         import torch
         from fqe_amd import _lib, engine, settings
@@ -251,11 +272,12 @@ class EngineRun:
         dev = _lib.device()
         self.c = torch.from_numpy(coeff_np).to(dev).to(torch_dtype)
         self.out = torch.empty_like(self.c)
-        saved = settings.fused
+        saved = settings.fused, settings.contraction
         settings.fused = fused
+        settings.contraction = contraction
         self.prec = engine.precision_of(torch_dtype)
         self.s, self.keep = engine.sector_struct(graph, op, self.prec)
-        settings.fused = saved
+        settings.fused, settings.contraction = saved
         self.opc = op.c_struct()
         self.sref = ctypes.byref(self.s)
         self.oref = ctypes.byref(self.opc)
@@ -327,8 +349,13 @@ class EngineRun:
             L, s = self._lib, self.s
             ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
             ev[0].record()
+            work = self.rowacc_ptr + 2 * self.n * 8   # row maxima, digits
             for _ in range(steps):
-                if self.prec == L.C128:
+                if self.prec == L.C128 and s.hbb_oz:
+                    L.call("fqehip_beta_gemm_digits", s.lena, s.lenb,
+                           s.hbb_oz, L.ptr(self.c), work, self.rowacc_ptr,
+                           None, L.stream_ptr())
+                elif self.prec == L.C128:
                     L.call("fqehip_beta_gemm_f64", s.lena, s.lenb, s.hbb_f64,
                            L.ptr(self.c), self.rowacc_ptr, None,
                            L.stream_ptr())
@@ -351,7 +378,7 @@ class EngineRun:
         self.torch.cuda.empty_cache()
 
 
-def kernel_work(op, graph, n, esz, mixed):
+def kernel_work(op, graph, n, esz, mixed, ozaki=False):
     """Algorithmic bytes and flops of one launch of each kernel.  GEMM
     flops are those of the real (npair+1) x npair pair matrix on 2n real
     columns; the mixed engine executes three BF16 products per flop.
@@ -360,7 +387,11 @@ def kernel_work(op, graph, n, esz, mixed):
     other strings plus the folded diagonal row (all npair + 1 rows with the
     all-rows kernel), by npair + 1 columns (extra h1' column); the
     beta-beta GEMM is lena x lenb x lenb complex times real (bfloat16
-    hi/lo copies of Hbb for the mixed engine, FP64 columns for FP64)."""
+    hi/lo copies of Hbb for the mixed engine, FP64 columns for FP64).
+    The Ozaki-digit engine moves the FP64 engine's bytes; its executed
+    int8 operations ("ops") are 15 digit products of the padded tiles: 64
+    live rows x 128 gathered rows per column (alpha rows) and 2 lena x
+    ldk x lenb (beta-beta GEMM, ldk = lenb rounded up to 64)."""
     from fqe_amd import settings
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
@@ -368,6 +399,16 @@ def kernel_work(op, graph, n, esz, mixed):
     rows = p + 1 if (mixed and settings.split_tile) else nslot + 1
     lenb = graph.lenb()
     ldh = ((lenb + 127) // 128) * 128
+    ldk = ((lenb + 63) // 64) * 64
+    if ozaki:
+        lena = graph.lena()
+        return {
+            "beta_gemm": {"bytes": 2 * 2 * n * esz + 5 * lenb * ldk,
+                          "ops": 15 * 2.0 * (2 * lena) * ldk * lenb},
+            "alpha_rows": {"bytes": (nslot + 3) * 2 * n * esz,
+                           "ops": 15 * 2.0 * 64 * 128 * 2 * n},
+            "slot_reduce": {"bytes": (nslot + 2) * 2 * n * esz},
+        }
     return {
         "beta_gemm": {"bytes": 2 * 2 * n * esz + (2 * ldh * ldh * 2 if mixed
                                                   else lenb * lenb * 8),
@@ -417,6 +458,24 @@ def roofline_entry(name, per_ms, work, mode, traffic=None):
     the measured HBM bytes per launch (rocprofv3 counters), if known."""
     t = per_ms[name] * 1e-3
     w = work[name]
+    if mode == "ozaki":
+        # int8 digit products (executed operations) against the int8 MFMA
+        # peak, or the FP64 engine's algorithmic bytes against HBM
+        t_hbm = w["bytes"] / (HBM_PEAK_GBS * 1e9)
+        t_mfma = w.get("ops", 0.0) / (PEAK_TFLOPS["i8"] * 1e12)
+        if t_mfma > t_hbm:
+            ach = w["ops"] / t / 1e12
+            return {"kernel": name, "bound": "mfma", "achieved": round(ach, 1),
+                    "peak": PEAK_TFLOPS["i8"], "unit": "TOP/s (int8)",
+                    "frac": round(ach / PEAK_TFLOPS["i8"], 4),
+                    "traffic": traffic, "ops_per_launch": w["ops"],
+                    "algorithmic_bytes_per_launch": w["bytes"]}
+        ach = w["bytes"] / t / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_launch": w["bytes"],
+                "int8_ops_per_launch": w.get("ops")}
     peak_fl = PEAK_TFLOPS["f64"] if mode == "fp64" else PEAK_TFLOPS["bf16"]
     mult = 1.0 if mode == "fp64" else 3.0
     t_hbm = w["bytes"] / (HBM_PEAK_GBS * 1e9)
@@ -436,24 +495,31 @@ def roofline_entry(name, per_ms, work, mode, traffic=None):
 
 
 def host_boundary_rate(coeff, h1, h2, norb, na, nb, reps=3):
-    """Mixed-precision applies/s when the caller hands over and takes back
+    """Headline-engine applies/s when the caller hands over and takes back
     host arrays (Wavefunction.set_wfn from numpy, apply, get_coeff): the
     PCIe-inclusive rate of the Python boundary."""
     import numpy
     import torch
     import fqe_amd as fqe
+    from fqe_amd import settings
     key = (na + nb, na - nb)
-    ham = fqe.get_restricted_hamiltonian((h1, h2))
-    wfn = fqe.Wavefunction([[na + nb, na - nb, norb]], dtype=numpy.complex64)
-    times = []
-    for r in range(reps + 1):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        wfn.set_wfn(strategy="from_data", raw_data={key: coeff})
-        wfn.apply(ham).get_coeff(key)
-        torch.cuda.synchronize()
-        if r:
-            times.append(time.perf_counter() - t0)
+    saved = settings.contraction
+    settings.contraction = "ozaki"
+    try:
+        ham = fqe.get_restricted_hamiltonian((h1, h2))
+        wfn = fqe.Wavefunction([[na + nb, na - nb, norb]],
+                               dtype=numpy.complex128)
+        times = []
+        for r in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            wfn.set_wfn(strategy="from_data", raw_data={key: coeff})
+            wfn.apply(ham).get_coeff(key)
+            torch.cuda.synchronize()
+            if r:
+                times.append(time.perf_counter() - t0)
+    finally:
+        settings.contraction = saved
     del wfn, ham
     torch.cuda.empty_cache()
     return reps / sum(times)
@@ -556,7 +622,7 @@ def set_main(args):
     }
     if not args.no_cpu_baseline:
         from oracle import fqe_set_oracle as so
-        cores = len(os.sched_getaffinity(0))
+        cores = HOST_CORES
         sg = so.SetGraph(nele, norb)
         coeffs, off = {}, 0
         for k in sg.keys:
@@ -578,35 +644,155 @@ def set_main(args):
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(coeff, h1, h2, norb, na, nb):
-    """One apply with the reference's compiled lm kernels on all host
-    cores (complex128)."""
-    from oracle import ref_lm
-    if not ref_lm.available():
-        return None, None
-    import platform
-    cores = len(os.sched_getaffinity(0))
-    graph = ref_lm.RefGraph(na, nb, norb)
-    t0 = time.perf_counter()
-    out = ref_lm.apply_12_lm(coeff, h1, h2, graph)
-    dt = time.perf_counter() - t0
-    cpu = "unknown"
+def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    cpu = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {"value": round(1.0 / dt, 5), "unit": "applies/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-            "kind": "reference",
-            "sample": (f"1 apply of the same workload (norb={norb}, "
-                       f"{na}a{nb}b) by the reference's compiled C lm "
-                       f"kernels in complex128 (OpenMP, scipy BLAS zaxpy), "
-                       f"{dt:.2f} s; host CPU {cpu} "
-                       f"({platform.machine()})")}, out
+    return "unknown"
+
+
+def cpu_baseline(coeff, h1, h2, norb, na, nb, repeats=5):
+    """The reference's compiled lm kernels (complex128) on the host cores:
+    one warm-up apply, then the median of ``repeats`` applies with every
+    core (OMP_PROC_BIND=close); plus one single-thread apply of the norb=12
+    6a6b sector (seed 1) as a per-core record."""
+    from oracle import ref_lm
+    if not ref_lm.available():
+        return None, None
+    import platform
+    import statistics
+    import numpy
+    import threadpoolctl
+    from fqe_amd import synthetic
+    threads = int(os.environ.get("OMP_NUM_THREADS", HOST_CORES))
+    graph = ref_lm.RefGraph(na, nb, norb)
+    out = ref_lm.apply_12_lm(coeff, h1, h2, graph)            # warm-up
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        out = ref_lm.apply_12_lm(coeff, h1, h2, graph)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    c12, h1s, h2s = synthetic.workload(12, 6, 6, 1, 12)
+    g12 = ref_lm.RefGraph(6, 6, 12)
+    with threadpoolctl.threadpool_limits(1):
+        ref_lm.apply_12_lm(c12, h1s, h2s, g12)
+        t0 = time.perf_counter()
+        ref_lm.apply_12_lm(c12, h1s, h2s, g12)
+        t1 = time.perf_counter() - t0
+    return {"value": round(1.0 / med, 5), "unit": "applies/s",
+            "cores": threads, "kind": "reference",
+            "sample": (f"median of {repeats} applies (after 1 warm-up) of the "
+                       f"same workload (norb={norb}, {na}a{nb}b) by the "
+                       f"reference's compiled C lm kernels in complex128 "
+                       f"(OpenMP {threads} threads, OMP_PROC_BIND="
+                       f"{os.environ.get('OMP_PROC_BIND')}, scipy BLAS "
+                       f"zaxpy, {ref_lm.ISA}); runs "
+                       f"{', '.join(f'{t:.2f}' for t in times)} s; host CPU "
+                       f"{_cpu_model()} ({platform.machine()})"),
+            "single_thread": {"value": round(1.0 / t1, 4),
+                              "unit": "applies/s",
+                              "sample": ("1 apply of norb=12 6a6b (seed 1) "
+                                         "on one thread")}}, out
+
+
+def _ref_taylor(coeff, h1, h2, graph, time_, accuracy=1e-15, expansion=30):
+    """The reference's Taylor loop (src/fqe/wavefunction.py:556-568) over its
+    compiled apply, with the tensors of iht(t) = -i t (h1, h2)."""
+    import math
+    import numpy
+    from oracle import ref_lm
+    a1, a2 = -1j * time_ * h1, -1j * time_ * h2
+    work = coeff.copy()
+    result = coeff.copy()
+    for order in range(1, expansion):
+        work = ref_lm.apply_12_lm(work, a1, a2, graph)
+        c = 1.0 / math.factorial(order)
+        result += c * work
+        if numpy.linalg.norm(work) * c < accuracy:
+            return result, order
+    raise RuntimeError("maximum taylor expansion limit reached")
+
+
+def cfg1_line(with_cpu, chain=100, cpu_chain=10):
+    """norb=12 6a6b (853,776 determinants), seed 1, 12 factors: ``chain``
+    chained applies C <- H C / ||H C|| plus time_evolve(0.05) of the start
+    vector, FP64 on the GPU (Wavefunction API, C resident in HBM); the
+    reference's compiled C on the host times the first ``cpu_chain`` links
+    (median per apply, scaled to ``chain``) and the whole Taylor series."""
+    import statistics
+    import numpy
+    import torch
+    import fqe_amd as fqe
+    from fqe_amd import synthetic
+    c, h1, h2 = synthetic.workload(12, 6, 6, 1, 12)
+    key = (12, 0)
+    ham = fqe.get_restricted_hamiltonian((h1, h2))
+    wfn = fqe.Wavefunction([[12, 0, 12]])
+    wfn.set_wfn(strategy="from_data", raw_data={key: c})
+    x = wfn.apply(ham)            # warm-up: tables, operator, kernels
+    x.normalize()
+    wfn.time_evolve(0.05, ham)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x = wfn
+    links = []
+    for _ in range(chain):
+        x = x.apply(ham)
+        x.normalize()
+        if len(links) < cpu_chain:
+            links.append(x)
+    torch.cuda.synchronize()
+    t_chain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ev = wfn.time_evolve(0.05, ham)
+    torch.cuda.synchronize()
+    t_evolve = time.perf_counter() - t0
+    line = {"workload": ("norb=12 6a6b (853,776 determinants), seed 1, 12 "
+                         f"factors: {chain} chained applies (normalised) + "
+                         "time_evolve(0.05), Taylor, accuracy 1e-15"),
+            "gpu_fp64_s": round(t_chain + t_evolve, 4),
+            "gpu_fp64_applies_per_s": round(chain / t_chain, 2),
+            "gpu_fp64_time_evolve_s": round(t_evolve, 4),
+            "taylor_terms": int(ev._taylor_terms)}
+    if not with_cpu:
+        return line
+    from oracle import ref_lm
+    if not ref_lm.available():
+        return line
+    g = ref_lm.RefGraph(6, 6, 12)
+    y = c.copy()
+    per = []
+    errs = []
+    for k in range(cpu_chain):
+        t1 = time.perf_counter()
+        y = ref_lm.apply_12_lm(y, h1, h2, g)
+        y /= numpy.linalg.norm(y)
+        if k:                        # the first link warms the caches
+            per.append(time.perf_counter() - t1)
+        errs.append(float(numpy.linalg.norm(links[k].get_coeff(key) - y)))
+    t1 = time.perf_counter()
+    ref_ev, nterms = _ref_taylor(c, h1, h2, g, 0.05)
+    t_cpu_ev = time.perf_counter() - t1
+    cpu_s = statistics.median(per) * chain + t_cpu_ev
+    gev = ev.get_coeff(key)
+    line.update({
+        "cpu_reference_s": round(cpu_s, 3),
+        "cpu_reference_time_evolve_s": round(t_cpu_ev, 3),
+        "cpu_sample": (f"{cpu_chain - 1} chained applies timed (median per "
+                       f"apply x {chain}) + the whole Taylor series, "
+                       f"{HOST_CORES} host cores"),
+        "speedup_gpu_vs_cpu": round(cpu_s / (t_chain + t_evolve), 1),
+        "max_rel_err_chain_links": max(errs),
+        "rel_err_time_evolve": float(numpy.linalg.norm(gev - ref_ev) /
+                                     numpy.linalg.norm(ref_ev)),
+        "reference_taylor_terms": int(nterms),
+        "unitarity": float(abs(numpy.linalg.norm(gev) - 1.0))})
+    return line
 
 
 def main():
@@ -652,49 +838,56 @@ def main():
     op = engine.prepare_operator((h1, h2), norb, dev)
     n = coeff.size
 
-    results = {}
-    outs = {}
-    modes = [m for m in ("fp64", "mixed") if m in args.modes.split(",")]
-    if modes != ["fp64", "mixed"]:
-        for mode in modes:
-            dt = torch.complex128 if mode == "fp64" else torch.complex64
-            run = EngineRun(graph, op, coeff, dt, fused=not args.unfused)
-            wall, per = run.run(args.steps, args.warmup)
-            print(json.dumps({"mode": mode, "applies_per_s": args.steps / wall,
-                              "kernels_ms": per}))
-            run.free()
-        return
-    for mode, dt in (("fp64", torch.complex128), ("mixed", torch.complex64)):
-        run = EngineRun(graph, op, coeff, dt, fused=not args.unfused)
+    engines = {"fp64": (torch.complex128, "fp64"),
+               "ozaki": (torch.complex128, "ozaki"),
+               "bf16x3": (torch.complex64, "fp64")}
+    modes = [m for m in engines if m in args.modes.split(",")]
+    results, outs = {}, {}
+    for mode in modes:
+        dt, contraction = engines[mode]
+        run = EngineRun(graph, op, coeff, dt, fused=not args.unfused,
+                        contraction=contraction)
         wall, per = run.run(args.steps, args.warmup)
         outs[mode] = run.result()
         results[mode] = {"wall": wall, "per": per, "fused": run.fused,
-                         "esz": 8 if dt == torch.complex128 else 4}
+                         "tile": int(run.s.tile), "digits_beta":
+                         bool(run.s.hbb_oz)}
         print(f"{mode}: {args.steps / wall:.2f} applies/s "
               f"{ {k: round(v, 3) for k, v in per.items()} }",
               file=sys.stderr, flush=True)
         run.free()
         del run
+    if set(modes) != set(engines):
+        for mode in modes:
+            print(json.dumps({"mode": mode,
+                              "applies_per_s": args.steps /
+                              results[mode]["wall"],
+                              "kernels_ms": results[mode]["per"]}))
+        return
 
     def rel(a, b):
         return float(numpy.linalg.norm(a - b) / numpy.linalg.norm(b))
 
-    err_mixed_fp64 = rel(outs["mixed"], outs["fp64"])
-    host_rate = host_boundary_rate(coeff, h1, h2, norb, na, nb)
-
     cpu, ref_out = (None, None)
     if not args.no_cpu_baseline:
         cpu, ref_out = cpu_baseline(coeff, h1, h2, norb, na, nb)
+    cfg1 = None
+    if not args.no_cfg1 and (norb, na, nb) == (14, 7, 7):
+        cfg1 = cfg1_line(not args.no_cpu_baseline)
+    host_rate = host_boundary_rate(coeff, h1, h2, norb, na, nb)
 
-    mixed = results["mixed"]
-    fp64 = results["fp64"]
-    value = args.steps / mixed["wall"]
-    fp64_value = args.steps / fp64["wall"]
-    work_m = kernel_work(op, graph, n, 4, True)
+    oz, fp, bf = results["ozaki"], results["fp64"], results["bf16x3"]
+    value = args.steps / oz["wall"]
+    fp64_value = args.steps / fp["wall"]
+    bf_value = args.steps / bf["wall"]
+    work_o = kernel_work(op, graph, n, 8, False, ozaki=True)
     work_d = kernel_work(op, graph, n, 8, False)
-    dom_m = max(mixed["per"], key=mixed["per"].get)
-    dom_d = max(fp64["per"], key=fp64["per"].get)
+    work_m = kernel_work(op, graph, n, 4, True)
+    dom_o = max(oz["per"], key=oz["per"].get)
+    dom_d = max(fp["per"], key=fp["per"].get)
+    dom_m = max(bf["per"], key=bf["per"].get)
     traffic = measured_traffic(norb, na, nb)
+    err_oz = rel(outs["ozaki"], outs["fp64"])
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -702,35 +895,52 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * mixed["wall"] / args.steps, 3),
+        "ms_per_step": round(1e3 * oz["wall"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 storage, bf16x3 MFMA, f32 accumulate",
+        "dtype": ("c128 storage; int8 digit products on MFMA (Ozaki: A' 5 "
+                  "digits, D' and C 48-bit integers, 15 digit pairs), exact "
+                  "int32 accumulation, FP64 assembly"),
         "data": "synthetic",
         "config": {"workload": (f"RestrictedHamiltonian apply, norb={norb} "
                                 f"{na}a{nb}b ({n} determinants), seed "
                                 f"{args.seed}, {norb} factors"),
                    "pair_basis": "compressed" if op.compressed else "ordered",
                    "npair": op.npair,
-                   "kernels": "fused" if mixed["fused"] else "unfused"},
-        "fp64": {"value": round(fp64_value, 3), "unit": "applies/s",
-                 "ms_per_step": round(1e3 * fp64["wall"] / args.steps, 3),
-                 "kernels_ms": {k: round(v, 4) for k, v in fp64["per"].items()},
-                 "roofline": roofline_entry(
-                     dom_d, fp64["per"], work_d, "fp64",
-                     traffic.get(dom_d + "_fp64"))},
+                   "kernels": "split" if oz["fused"] else "unfused",
+                   "digit_beta_gemm": oz["digits_beta"]},
+        "rel_err_mixed_vs_fp64_gpu": err_oz,
+        "gate_1e-9_met": err_oz <= 1e-9,
         "speedup_mixed_vs_fp64": round(value / fp64_value, 3),
+        "kernels_ms": {k: round(v, 4) for k, v in oz["per"].items()},
+        "roofline": roofline_entry(dom_o, oz["per"], work_o, "ozaki",
+                                   traffic.get(dom_o + "_ozaki")),
+        "fp64": {"value": round(fp64_value, 3), "unit": "applies/s",
+                 "ms_per_step": round(1e3 * fp["wall"] / args.steps, 3),
+                 "kernels_ms": {k: round(v, 4) for k, v in fp["per"].items()},
+                 "roofline": roofline_entry(
+                     dom_d, fp["per"], work_d, "fp64",
+                     traffic.get(dom_d + "_fp64"))},
+        "bf16x3_c64": {
+            "note": ("complex64 storage, BF16 hi/lo products with FP32 "
+                     "accumulation: outside the 1e-9 gate"),
+            "value": round(bf_value, 3), "unit": "applies/s",
+            "ms_per_step": round(1e3 * bf["wall"] / args.steps, 3),
+            "rel_err_vs_fp64_gpu": rel(outs["bf16x3"], outs["fp64"]),
+            "kernels_ms": {k: round(v, 4) for k, v in bf["per"].items()},
+            "roofline": roofline_entry(dom_m, bf["per"], work_m, "mixed",
+                                       traffic.get(dom_m))},
         "host_boundary_applies_per_s": round(host_rate, 3),
-        "rel_err_mixed_vs_fp64_gpu": err_mixed_fp64,
-        "kernels_ms": {k: round(v, 4) for k, v in mixed["per"].items()},
-        "roofline": roofline_entry(dom_m, mixed["per"], work_m, "mixed",
-                                   traffic.get(dom_m)),
         "cpu_baseline": cpu,
     }
     if ref_out is not None:
+        line["rel_err_mixed_vs_reference"] = rel(outs["ozaki"], ref_out)
         line["rel_err_fp64_vs_reference"] = rel(outs["fp64"], ref_out)
-        line["rel_err_mixed_vs_reference"] = rel(outs["mixed"], ref_out)
+        line["bf16x3_c64"]["rel_err_vs_reference"] = rel(outs["bf16x3"],
+                                                         ref_out)
+    if cfg1 is not None:
+        line["cfg1"] = cfg1
     print(json.dumps(line))
 
 

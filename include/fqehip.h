@@ -189,6 +189,10 @@ typedef struct fqehip_sector {
    * FP64 row kernel over the live pair rows); NULL keeps the FP64 row
    * kernel that also scatters beta excitations. */
   const double* hbb_f64;
+  /* Digits of the same FP64 beta-beta block for the Ozaki-digit split apply
+   * (tile FQEHIP_TILE_OZAKI; fqehip_beta_block_digits): when set, the
+   * beta-beta GEMM also runs as digit products; NULL keeps it in FP64. */
+  const void* hbb_oz;
 } fqehip_sector;
 
 #define FQEHIP_TILE_ROWS4 4
@@ -199,6 +203,14 @@ typedef struct fqehip_sector {
 /* complex128 split path (hbb_f64 set): the row kernel with a shared,
  * double-buffered D tile instead of column-owned waves (the default) */
 #define FQEHIP_TILE_SPLIT_F64_DTILE 32
+/* complex128 split path (hbb_f64 set): the alpha-row contraction as exact
+ * products of 8-bit digits on the int8 matrix cores (Ozaki scheme: A' in
+ * five balanced digits per pair row, D' as 48-bit integers per alpha
+ * string, 15 digit pairs, int32 accumulation, FP64 assembly); complex128
+ * storage and FP64 slot rows as on the FP64 path.  The workspace grows by
+ * lena doubles (row maxima of |C|) and, with hbb_oz set, by the C digit
+ * planes of the beta-beta GEMM (5 * 2 lena * ldk bytes + lena doubles). */
+#define FQEHIP_TILE_OZAKI 64
 
 typedef struct fqehip_operator {
   int complex_op;   /* 0: A is (npair+1) x npair; 1: 2(npair+1) x 2npair */
@@ -243,6 +255,28 @@ int64_t fqehip_beta_block_ld(int64_t lenb);
 int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b,
                       const double* d_a, double* d_hcol, uint16_t* d_hhi,
                       uint16_t* d_hlo, int64_t ldh, void* stream);
+/* Ozaki digits of an FP64 beta-beta block (d_hcol, the hcol output of
+ * fqehip_beta_block) for the digit apply's beta-beta GEMM, written to d_hoz
+ * (fqehip_beta_block_digits_bytes(lenb) bytes): int8 [5][lenb][ldk]
+ * (ldk = lenb rounded up to 64; per column Ib' of the GEMM, i.e. row Ib'
+ * of Hbb, scaled by 2^eh with |Hbb 2^eh| < 2^38 and rounded to integers z,
+ * the five exact balanced digits of z, weights 2^32, 2^24, ..., 1), then
+ * double [lenb] scales 2^-eh, then double [lenb] offset corrections of
+ * the C digits (128 (2^8 + 2^16 + 2^24 + 2^32) sum of the column's z).
+ * Once per operator and beta string count; lenb < 26214. */
+int64_t fqehip_beta_block_digits_bytes(int64_t lenb);
+int fqehip_beta_block_digits(int64_t lenb, const double* d_hcol, void* d_hoz,
+                             void* stream);
+/* d_out[Ia][Ib'] = sum_Ib C[Ia][Ib] Hbb[Ib'][Ib] for complex128 C as digit
+ * products (the beta-beta half of the Ozaki-digit split apply, for callers
+ * that time or schedule it alone): row maxima of |C| into d_work (lena
+ * doubles), the C digit planes after them (5 * 2 lena * ldk bytes + lena
+ * doubles), then the int8 GEMM.  Replaces, like fqehip_beta_gemm_f64, the
+ * same-spin beta part of lm_apply_array12_same_spin_opt
+ * (src/fqe/lib/fqe_data.c:668-720). */
+int fqehip_beta_gemm_digits(int64_t lena, int64_t lenb, const void* d_hoz,
+                            const void* d_coeff, void* d_work, void* d_out,
+                            const int32_t* d_done, void* stream);
 /* d_out[Ia][Ib'] = sum_Ib d_coeff[Ia][Ib] Hbb[Ib'][Ib] for complex64
  * d_coeff / d_out ([lena][lenb] interleaved), BF16x3 products with FP32
  * accumulation.  The opposite-spin and alpha parts of the split apply
@@ -309,6 +343,27 @@ int fqehip_apply_noalloc(int precision, const fqehip_sector* s,
                          double* d_acc, double* d_norm_partials,
                          const int32_t* d_done, void* stream);
 
+/* fqehip_apply on the unfused gather / GEMM / scatter kernels in panels of
+ * orbital pairs, for sectors whose dvec and Y (2 npair + 1 planes of
+ * 16 lena lenb bytes each) outgrow device memory: per panel of output pairs
+ * (the h1' row included), the input pairs are gathered and contracted
+ * panel by panel into that panel's Y, which is then scattered into an FP64
+ * accumulator; a last pass applies alpha, beta and the epilogue of
+ * fqehip_scatter.  Panel sizes follow workspace_bytes (at least 3 planes:
+ * the accumulator, one input and one output pair); the whole dvec is
+ * gathered once when it fits, else once per output panel.  complex128,
+ * two-body operators (real or complex).  Replaces the dvec path of
+ * src/fqe/fqe_data.py:646-683 (the reference has no chunked variant: its
+ * dvec path allocates the whole dvec, :2209-2234, and its C lm kernels
+ * avoid the dvec, src/fqe/lib/fqe_data.c:668-996). */
+int fqehip_apply_panels(int precision, const fqehip_sector* s,
+                        const fqehip_operator* op, const void* d_coeff,
+                        double alpha_re, double alpha_im, double beta_re,
+                        double beta_im, const void* d_cin, void* d_out,
+                        double* d_acc, double* d_norm_partials,
+                        const int32_t* d_done, void* d_workspace,
+                        int64_t workspace_bytes, void* stream);
+
 /* Taylor series psi(t) = sum_k (-i t H)^k psi / k! for a real operator
  * (time folded into the scalar), reference
  * src/fqe/wavefunction.py:556-568.  d_term (storage precision) holds psi
@@ -323,6 +378,40 @@ int fqehip_taylor(int precision, const fqehip_sector* s,
                   int expansion, void* d_term, double* d_result,
                   int32_t* d_status, double* d_partials, double* d_norm,
                   void* d_workspace, void* stream);
+
+/* One step of the Chebyshev propagator's three-term recurrence, after the
+ * apply that wrote (2/a) H T_(k-1) - T_(k-2) into d_tk (fqehip_apply with
+ * alpha = 2/a, beta = -1):
+ *   T_k += gamma T_(k-1) (skipped when d_tkm1 is NULL);
+ *   result += c T_k (complex128);
+ *   d_norm_partials[block] = sum of |T_k|^2 (optional; the blocks of
+ *   fqehip_norm_partials_count(n), for fqehip_norm_finalize).
+ * Does nothing when d_done and *d_done.  Replaces the per-term axpys of the
+ * reference's Chebyshev branch (src/fqe/wavefunction.py:570-602), which
+ * make four passes over host wavefunctions per term. */
+int fqehip_cheb_update(int precision, int64_t n, void* d_tk,
+                       const void* d_tkm1, double gamma_re, double gamma_im,
+                       double c_re, double c_im, double* d_result,
+                       double* d_norm_partials, const int32_t* d_done,
+                       void* stream);
+
+/* The series of fqehip_taylor recorded once into a HIP graph (on a private
+ * stream; every apply, norm reduction and device-flag test of the
+ * expansion - 1 terms is a node) and replayed with one launch: one host
+ * call per time_evolve instead of one per kernel.  Buffers are referenced,
+ * not copied: each launch starts from what d_term holds and adds into
+ * d_result (fill both before a launch, as for fqehip_taylor), and writes
+ * d_status.  The reference loops over its terms in Python
+ * (src/fqe/wavefunction.py:556-568). */
+typedef struct fqehip_taylor_graph fqehip_taylor_graph;
+int fqehip_taylor_graph_create(int precision, const fqehip_sector* s,
+                               const fqehip_operator* op, double time,
+                               double accuracy, int expansion, void* d_term,
+                               double* d_result, int32_t* d_status,
+                               double* d_partials, double* d_norm,
+                               void* d_workspace, fqehip_taylor_graph** out);
+int fqehip_taylor_graph_launch(fqehip_taylor_graph* graph, void* stream);
+void fqehip_taylor_graph_destroy(fqehip_taylor_graph* graph);
 
 /* ---------------------------------------------------------------------
  * Davidson eigensolver kernels (the reference's davidsonliu_fqe,

@@ -93,9 +93,9 @@ k_dvec_gather(int64_t lena, int64_t lenb, double inv_lenb,
 constexpr int kGemmRowTiles = 8;
 
 __global__ void __launch_bounds__(kBlock)
-k_gemm_f64(int m, int k, const double* __restrict__ a, int64_t ncol,
+k_gemm_f64(int m, int k, const double* __restrict__ a, int lda, int64_t ncol,
            const double* __restrict__ b, double* __restrict__ y,
-           const int32_t* __restrict__ done) {
+           int accumulate, const int32_t* __restrict__ done) {
   if (done && *done) return;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -118,7 +118,7 @@ k_gemm_f64(int m, int k, const double* __restrict__ a, int64_t ncol,
     for (int t = 0; t < kGemmRowTiles; ++t) {
       if (t < ntiles) {
         const int r = row0 + t * 16 + (lane & 15);
-        const double av = (r < m && kok) ? a[static_cast<int64_t>(r) * k + kk] : 0.0;
+        const double av = (r < m && kok) ? a[static_cast<int64_t>(r) * lda + kk] : 0.0;
         acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
       }
     }
@@ -130,7 +130,10 @@ k_gemm_f64(int m, int k, const double* __restrict__ a, int64_t ncol,
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = row0 + t * 16 + (lane >> 4) + 4 * i;
-        if (r < m) y[static_cast<int64_t>(r) * ncol + col] = acc[t][i];
+        if (r < m) {
+          double* dst = y + static_cast<int64_t>(r) * ncol + col;
+          *dst = accumulate ? *dst + acc[t][i] : acc[t][i];
+        }
       }
     }
   }
@@ -261,6 +264,67 @@ k_apply1(int npair, int64_t lena, int64_t lenb, double inv_lenb,
   if (partials) block_sum_store(sq, partials);
 }
 
+// Chunked apply (fqehip_apply_panels): the scatter of one panel of output
+// pairs [p0, p0 + np) into an FP64 accumulator, oacc (+)= sum_p E_p Y[p];
+// the h1' row (p == npair) is added without excitation.  Y planes of the
+// panel: 2n values per pair (re | im halves), as in the full Y.
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_scatter_panel(int p0, int np, int npair, int64_t lena, int64_t lenb,
+                double inv_lenb, const int32_t* __restrict__ pta,
+                const int32_t* __restrict__ ptb, const T* __restrict__ y,
+                double* __restrict__ oacc, int first,
+                const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t n = lena * lenb;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  int64_t ia, ib;
+  split_index(idx, lenb, inv_lenb, ia, ib);
+  double re = first ? 0.0 : oacc[2 * idx];
+  double im = first ? 0.0 : oacc[2 * idx + 1];
+  for (int j = 0; j < np; ++j) {
+    const int p = p0 + j;
+    const T* yre = y + static_cast<int64_t>(2 * j) * n;
+    if (p == npair) {
+      re += static_cast<double>(yre[idx]);
+      im += static_cast<double>(yre[n + idx]);
+      continue;
+    }
+    const int32_t a = pta[static_cast<int64_t>(p) * lena + ia];
+    if (a) {
+      const int64_t off = ((a > 0 ? a : -a) - 1) * lenb + ib;
+      re += signed_val(a, static_cast<double>(yre[off]));
+      im += signed_val(a, static_cast<double>(yre[n + off]));
+    }
+    const int32_t b = ptb[static_cast<int64_t>(p) * lenb + ib];
+    if (b) {
+      const int64_t off = ia * lenb + (b > 0 ? b : -b) - 1;
+      re += signed_val(b, static_cast<double>(yre[off]));
+      im += signed_val(b, static_cast<double>(yre[n + off]));
+    }
+  }
+  oacc[2 * idx] = re;
+  oacc[2 * idx + 1] = im;
+}
+
+// Final pass of the chunked apply: out = alpha oacc + beta cin, with the
+// epilogue of fqehip_scatter (acc, norm partials).
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+k_panel_epilogue(int64_t n, const double* __restrict__ oacc, double alpha_re,
+                 double alpha_im, double beta_re, double beta_im, const T* cin,
+                 T* out, double* __restrict__ acc, double* __restrict__ partials,
+                 const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (idx < n)
+    epilogue(idx, oacc[2 * idx], oacc[2 * idx + 1], alpha_re, alpha_im,
+             beta_re, beta_im, cin, out, acc, sq);
+  if (partials) block_sum_store(sq, partials);
+}
+
 // Single-workgroup reduction in a fixed order; optional convergence test.
 __global__ void __launch_bounds__(kBlock)
 k_norm_finalize(const double* __restrict__ partials, int64_t np,
@@ -375,10 +439,11 @@ int launch_apply1(int npair, int64_t lena, int64_t lenb, const int32_t* pta,
 
 int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
                     const double* b, double* y, hipStream_t st,
-                    const int32_t* done) {
+                    const int32_t* done, int lda = -1, int accumulate = 0) {
   if (m == 0 || ncol == 0) return FQEHIP_OK;
   dim3 grid(blocks_for(ncol, 64), (m + kGemmRowTiles * 16 - 1) / (kGemmRowTiles * 16));
-  k_gemm_f64<<<grid, kBlock, 0, st>>>(m, k, a, ncol, b, y, done);
+  k_gemm_f64<<<grid, kBlock, 0, st>>>(m, k, a, lda < 0 ? k : lda, ncol, b, y,
+                                      accumulate, done);
   FQEHIP_CHECK_LAUNCH("k_gemm_f64");
   return FQEHIP_OK;
 }
@@ -394,6 +459,21 @@ int launch_fused(int precision, int norb, int npair, int64_t lena, int64_t lenb,
                  const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
                  int nslot, const void* coeff, void* slots, void* rowacc,
                  bool split, int tile, hipStream_t st, const int32_t* done);
+// Defined in ozaki.hip.
+bool split_oz_supported(int norb, int npair, int64_t lena, int64_t lenb);
+int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
+                    const double* a, const int32_t* pta, const int32_t* ptb,
+                    const int32_t* fwa, const int32_t* sla, int nslot,
+                    const double* coeff, const double* rowmax, double* slots,
+                    double* rowacc, hipStream_t st, const int32_t* done);
+int launch_row_absmax(int64_t lena, int64_t lenb, const double* coeff,
+                      double* rowmax, hipStream_t st, const int32_t* done);
+bool beta_oz_shape_ok(int64_t lena, int64_t lenb);
+int64_t beta_oz_workspace_bytes(int64_t lena, int64_t lenb);
+int launch_beta_gemm_oz(int64_t lena, int64_t lenb, const void* hoz,
+                        const double* coeff, const double* rowmax,
+                        void* cwork, double* out, hipStream_t st,
+                        const int32_t* done);
 // Defined in beta_block.hip.
 bool beta_gemm_shape_ok(int64_t lena, int64_t lenb, int64_t ldh);
 int launch_beta_gemm(int64_t lena, int64_t lenb, int64_t ldh, const void* hhi,
@@ -589,12 +669,32 @@ int check_sector_op(int precision, const fqehip_sector* s,
   FQEHIP_CHECK_ARG(s->tile == 0 || s->tile == FQEHIP_TILE_ROWS4 ||
                        s->tile == FQEHIP_TILE_ROWS8 ||
                        s->tile == FQEHIP_TILE_SPLIT_ALLROWS ||
-                       s->tile == FQEHIP_TILE_SPLIT_F64_DTILE, "tile=%d", s->tile);
+                       s->tile == FQEHIP_TILE_SPLIT_F64_DTILE ||
+                       s->tile == FQEHIP_TILE_OZAKI, "tile=%d", s->tile);
   return FQEHIP_OK;
 }
 
 size_t elem_size(int precision) {
   return precision == FQEHIP_C128 ? sizeof(double) : sizeof(float);
+}
+
+// The complex128 split apply with Ozaki-digit contractions.
+bool ozaki_split(int precision, const fqehip_sector* s) {
+  return precision == FQEHIP_C128 && s->hbb_f64 != nullptr &&
+         s->tile == FQEHIP_TILE_OZAKI;
+}
+
+// Workspace of the fused / split paths: slot rows, row accumulators and,
+// for the Ozaki-digit split apply, the row maxima of |C| (lena doubles).
+int64_t fused_workspace_bytes(int precision, const fqehip_sector* s) {
+  const int64_t n = s->lena * s->lenb;
+  int64_t b = static_cast<int64_t>(s->nslot + 1) * 2 * n *
+              static_cast<int64_t>(elem_size(precision));
+  if (ozaki_split(precision, s)) {
+    b += s->lena * static_cast<int64_t>(sizeof(double));  // row maxima
+    if (s->hbb_oz) b += beta_oz_workspace_bytes(s->lena, s->lenb);
+  }
+  return b;
 }
 
 }  // namespace
@@ -604,6 +704,10 @@ extern "C" int fqehip_fused_supported(int precision, const fqehip_sector* s,
   if (!(s && op && op->a && !op->h1 && !op->complex_op && s->fwd_a &&
         s->fwd_b && s->slot_a))
     return 0;
+  if (ozaki_split(precision, s))
+    return split_oz_supported(s->norb, s->npair, s->lena, s->lenb) &&
+           (s->hbb_oz ? beta_oz_shape_ok(s->lena, s->lenb)
+                      : beta_gemm_f64_shape_ok(s->lena, s->lenb));
   if (precision == FQEHIP_C128 && s->hbb_f64)
     return split_f64_supported(s->npair, s->lena, s->lenb) &&
            beta_gemm_f64_shape_ok(s->lena, s->lenb);
@@ -625,6 +729,31 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
   char* ws = static_cast<char*>(d_workspace);
   void* rowacc = ws + static_cast<int64_t>(s->nslot) * 2 * n * elem_size(precision);
   hipStream_t st = as_stream(stream);
+  if (ozaki_split(precision, s)) {
+    // row maxima of |C| (digit scales), rowacc <- C Hbb^T (digit products
+    // when the operator's Hbb digits are set, else FP64), then the alpha
+    // rows (digit products)
+    double* rowmax = static_cast<double*>(rowacc) + 2 * n;
+    int rc = launch_row_absmax(s->lena, s->lenb,
+                               static_cast<const double*>(d_coeff), rowmax, st,
+                               d_done);
+    if (rc) return rc;
+    if (s->hbb_oz)
+      rc = launch_beta_gemm_oz(s->lena, s->lenb, s->hbb_oz,
+                               static_cast<const double*>(d_coeff), rowmax,
+                               rowmax + s->lena, static_cast<double*>(rowacc),
+                               st, d_done);
+    else
+      rc = launch_beta_gemm_f64(s->lena, s->lenb, s->hbb_f64,
+                                static_cast<const double*>(d_coeff),
+                                static_cast<double*>(rowacc), st, d_done);
+    if (rc) return rc;
+    return launch_split_oz(s->norb, s->npair, s->lena, s->lenb, op->a,
+                           s->ptab_a, s->ptab_b, s->fwd_a, s->slot_a, s->nslot,
+                           static_cast<const double*>(d_coeff), rowmax,
+                           reinterpret_cast<double*>(ws),
+                           static_cast<double*>(rowacc), st, d_done);
+  }
   if (precision == FQEHIP_C128 && s->hbb_f64 != nullptr) {
     // rowacc <- C Hbb^T (FP64), then the live alpha rows add their part
     int rc = launch_beta_gemm_f64(s->lena, s->lenb, s->hbb_f64,
@@ -679,10 +808,10 @@ extern "C" int64_t fqehip_apply_workspace_bytes(int precision,
                                                const fqehip_sector* s) {
   if (!s) return -1;
   const int64_t n = s->lena * s->lenb;
-  const int64_t unfused = static_cast<int64_t>(2 * s->npair + 1) * 2 * n;
-  const int64_t fused = static_cast<int64_t>(s->nslot + 1) * 2 * n;
-  return (unfused > fused ? unfused : fused) *
-         static_cast<int64_t>(elem_size(precision));
+  const int64_t unfused = static_cast<int64_t>(2 * s->npair + 1) * 2 * n *
+                          static_cast<int64_t>(elem_size(precision));
+  const int64_t fused = fused_workspace_bytes(precision, s);
+  return unfused > fused ? unfused : fused;
 }
 
 extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
@@ -772,5 +901,81 @@ extern "C" int fqehip_taylor(int precision, const fqehip_sector* s,
                               nterms, stream);
     if (rc) return rc;
   }
+  return FQEHIP_OK;
+}
+
+extern "C" int fqehip_apply_panels(int precision, const fqehip_sector* s,
+                                   const fqehip_operator* op,
+                                   const void* d_coeff, double alpha_re,
+                                   double alpha_im, double beta_re,
+                                   double beta_im, const void* d_cin,
+                                   void* d_out, double* d_acc,
+                                   double* d_norm_partials,
+                                   const int32_t* d_done, void* d_workspace,
+                                   int64_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int rc = check_sector_op(precision, s, op)) return rc;
+  FQEHIP_CHECK_ARG(precision == FQEHIP_C128,
+                   "the chunked apply runs in complex128");
+  FQEHIP_CHECK_ARG(op->a && !op->h1, "the chunked apply needs a two-body "
+                   "operator (fqehip_apply takes one-body operators)");
+  FQEHIP_CHECK_ARG(d_coeff && d_out && d_workspace, "NULL pointer");
+  FQEHIP_CHECK_ARG((beta_re == 0.0 && beta_im == 0.0) || d_cin,
+                   "beta != 0 needs d_cin");
+  const int64_t n = s->lena * s->lenb;
+  if (n == 0) return FQEHIP_OK;
+  const int npair = s->npair;
+  const int64_t plane = 2 * n;                 // doubles per pair plane
+  const int64_t planes = workspace_bytes / (plane * 8) - 1;   // oacc first
+  FQEHIP_CHECK_ARG(planes >= 2,
+                   "workspace of %lld bytes holds fewer than 3 planes of %lld "
+                   "bytes", static_cast<long long>(workspace_bytes),
+                   static_cast<long long>(plane * 8));
+  int ki = static_cast<int>(planes / 2 < npair ? planes / 2 : npair);
+  if (ki < 1) ki = 1;
+  int64_t left = planes - ki;
+  int mo = static_cast<int>(left < npair + 1 ? left : npair + 1);
+  const bool complex_op = op->complex_op != 0;
+  const int w = complex_op ? 2 : 1;            // GEMM rows / cols per pair
+  const int64_t ncol = complex_op ? n : 2 * n;
+  double* ws = static_cast<double*>(d_workspace);
+  double* oacc = ws;
+  double* dpan = ws + plane;
+  double* ypan = dpan + static_cast<int64_t>(ki) * plane;
+  const int lda = w * npair;
+  hipStream_t st = as_stream(stream);
+  const bool dall = ki >= npair;               // the whole dvec fits
+  if (dall) {
+    int rc = launch_gather<double>(npair, s->lena, s->lenb, s->ptab_a,
+                                   s->ptab_b, d_coeff, dpan, st, d_done);
+    if (rc) return rc;
+  }
+  for (int p0 = 0; p0 < npair + 1; p0 += mo) {
+    const int np = (npair + 1 - p0) < mo ? (npair + 1 - p0) : mo;
+    for (int q0 = 0; q0 < npair; q0 += ki) {
+      const int nq = (npair - q0) < ki ? (npair - q0) : ki;
+      if (!dall) {
+        int rc = launch_gather<double>(
+            nq, s->lena, s->lenb, s->ptab_a + static_cast<int64_t>(q0) * s->lena,
+            s->ptab_b + static_cast<int64_t>(q0) * s->lenb, d_coeff, dpan, st,
+            d_done);
+        if (rc) return rc;
+      }
+      const double* a = op->a + static_cast<int64_t>(w * p0) * lda + w * q0;
+      int rc = launch_gemm_f64(w * np, w * nq, a, ncol,
+                               dpan + (dall ? static_cast<int64_t>(q0) * plane : 0),
+                               ypan, st, d_done, lda, q0 > 0);
+      if (rc) return rc;
+    }
+    k_scatter_panel<double><<<blocks_for(n, kBlock), kBlock, 0, st>>>(
+        p0, np, npair, s->lena, s->lenb, 1.0 / s->lenb, s->ptab_a, s->ptab_b,
+        ypan, oacc, p0 == 0, d_done);
+    FQEHIP_CHECK_LAUNCH("k_scatter_panel");
+  }
+  k_panel_epilogue<double><<<blocks_for(n, kBlock), kBlock, 0, st>>>(
+      n, oacc, alpha_re, alpha_im, beta_re, beta_im,
+      static_cast<const double*>(d_cin), static_cast<double*>(d_out), d_acc,
+      d_norm_partials, d_done);
+  FQEHIP_CHECK_LAUNCH("k_panel_epilogue");
   return FQEHIP_OK;
 }

@@ -21,10 +21,13 @@
 // The dvec never reaches HBM: per apply the traffic is C (cached), the
 // slot rows (n_alpha (norb - n_alpha) per determinant), rowacc and out.
 #include "fqehip_common.h"
+#include "rows_common.h"
 
 
 namespace fqehip {
 namespace {
+
+using namespace rows;
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
@@ -43,37 +46,6 @@ __device__ __forceinline__ float from_bf16(__bf16 h) {
                          << 16);
 }
 
-__device__ __forceinline__ int32_t iabs(int32_t v) { return v < 0 ? -v : v; }
-
-// Raw buffer loads.  A buffer resource carries the byte size of the array;
-// the hardware returns 0 for an offset outside it, so absent table entries
-// are loaded with offset -1 instead of a branch around the load (all of a
-// thread's loads can then be in flight together).
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* p,
-                                                            int64_t bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0,
-                                           static_cast<int>(bytes), 0x00020000);
-}
-
-__device__ __forceinline__ int32_t bload_i32(__amdgpu_buffer_rsrc_t r, int off) {
-  return static_cast<int32_t>(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-
-__device__ __forceinline__ float2 bload_f2(__amdgpu_buffer_rsrc_t r, int off) {
-  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-  return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
-}
-
-__device__ __forceinline__ double2 bload_d2(__amdgpu_buffer_rsrc_t r, int off) {
-  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-  const uint64_t lo = static_cast<uint64_t>(v[0]) | (static_cast<uint64_t>(v[1]) << 32);
-  const uint64_t hi = static_cast<uint64_t>(v[2]) | (static_cast<uint64_t>(v[3]) << 32);
-  return make_double2(__longlong_as_double(static_cast<long long>(lo)),
-                      __longlong_as_double(static_cast<long long>(hi)));
-}
 
 __device__ __forceinline__ float sgnf(int32_t v) {
   return v > 0 ? 1.0f : (v < 0 ? -1.0f : 0.0f);
@@ -135,49 +107,6 @@ k_fill(int64_t n, int32_t v, int32_t* out) {
 // ---------------------------------------------------------------------------
 constexpr int kFWaves = 8;
 constexpr int kFThreads = kFWaves * 64;
-constexpr int kRowTabs = 128;              // per-row LDS tables (>= kMPad)
-
-struct RowTables {
-  int32_t* aoff;    // [q] offset of C row src_a(q) (elements), -1 if none
-  int32_t* asgn;    // [q] sign of the alpha gather entry
-  int64_t* sbase;   // [p] slot row base of the alpha target, -1 none, -2 diag
-  int32_t* ssgn;    // [p] sign of the alpha target
-};
-
-// own_q >= 0 (split kernel): gather row own_q is C[ia] itself, sign +1.
-template <typename T>
-__device__ __forceinline__ void load_row_tables(
-    int64_t ia, int K, int64_t lena, int64_t lenb, int nslot,
-    const int32_t* __restrict__ pta, const int32_t* __restrict__ fwa,
-    const int32_t* __restrict__ sla, RowTables rt, int tid, int nthreads,
-    int own_q = -1) {
-  for (int q = tid; q < kRowTabs; q += nthreads) {
-    int32_t off = -1, sg = 0;
-    int64_t sb = -1;
-    int32_t ss = 0;
-    if (q == own_q) {
-      off = static_cast<int32_t>(ia * lenb * 2);
-      sg = 1;
-    } else if (q < K) {
-      const int32_t a = pta[static_cast<int64_t>(q) * lena + ia];
-      if (a) {
-        off = static_cast<int32_t>((iabs(a) - 1) * lenb * 2);
-        sg = a > 0 ? 1 : -1;
-      }
-      const int32_t f = fwa[static_cast<int64_t>(q) * lena + ia];
-      if (f) {
-        const int64_t tgt = iabs(f) - 1;
-        ss = f > 0 ? 1 : -1;
-        sb = (tgt == ia) ? -2
-                         : (tgt * nslot + sla[static_cast<int64_t>(q) * lena + ia]) * lenb * 2;
-      }
-    }
-    rt.aoff[q] = off;
-    rt.asgn[q] = sg;
-    rt.sbase[q] = sb;
-    rt.ssgn[q] = ss;
-  }
-}
 
 // Scatter of one complex Y element (p, jb) of row ia (see file comment).
 template <typename T, typename A>
@@ -501,7 +430,6 @@ k_fused_bf16x3(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
 // chunk; ROWLDS stages row ia of C in LDS for the beta half of the gather
 // (random 8-byte reads within the row: from LDS instead of one cache line
 // per lane through L1/L2).
-constexpr int kLiveRows = 64;               // 4 tiles of 16 rows
 
 template <int NW, int WPE, int TBC, bool ROWLDS>
 __global__ void __launch_bounds__(NW * 64, WPE)

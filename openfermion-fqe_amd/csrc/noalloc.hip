@@ -33,8 +33,15 @@ int64_t needed_bytes(int precision, const fqehip_sector* s,
   if (op->h1) return 0;
   const int64_t n = s->lena * s->lenb;
   const int64_t elem = precision == FQEHIP_C128 ? 8 : 4;
-  if (fqehip_fused_supported(precision, s, op))
-    return static_cast<int64_t>(s->nslot + 1) * 2 * n * elem;
+  if (fqehip_fused_supported(precision, s, op)) {
+    int64_t b = static_cast<int64_t>(s->nslot + 1) * 2 * n * elem;
+    if (precision == FQEHIP_C128 && s->hbb_f64 && s->tile == FQEHIP_TILE_OZAKI) {
+      b += s->lena * 8;  // row maxima of |C|
+      if (s->hbb_oz)     // C digit planes and row scales
+        b += 5 * 2 * s->lena * ((s->lenb + 63) / 64 * 64) + 8 * s->lena;
+    }
+    return b;
+  }
   return static_cast<int64_t>(2 * s->npair + 1) * 2 * n * elem;
 }
 

@@ -1,0 +1,775 @@
+// Ozaki-digit variant of the complex128 split apply: the same operands,
+// layouts and scatter as the FP64 split path (complex128 C, FP64 slot rows
+// and row accumulators), with the pair contraction Y = A' D' of the alpha
+// row kernel computed as exact integer products of 8-bit digits on the
+// matrix cores (v_mfma_i32_16x16x64_i8, int32 accumulation) instead of
+// FP64 matrix-core products.
+//
+// Digits.  Per alpha string ia, A' (the live pair rows of the operator) is
+// scaled row by row by a power of two so that |A' 2^ea| < 2^38, rounded to
+// the integer z and split exactly into five balanced signed digits
+// a_0..a_4 in [-128, 127] of weights 2^32, 2^24, ..., 1 (from the low end:
+// d = ((z + 128) mod 256) - 128, z = (z - d) / 256).
+// D' (gathered from C) is scaled by one power of two per alpha string,
+// 2^ed with ed from a bound on |D'| (the largest row maximum of |C| over the
+// alpha sources of ia, plus twice the row maximum of row ia), so that
+// |D' 2^ed| < 2^46, and rounded to a 48-bit integer y.  Bytes 1..5 of y
+// (two's complement) are the digits d_1..d_5 of weights 2^8 .. 2^40; the
+// low byte is dropped (2^-38 of the bound).  d_1..d_4 are stored as
+// unsigned bytes minus 128 (one XOR), d_5 as the signed top byte, so no
+// carry propagation is needed; the 128 offsets are added back exactly, for
+// every digit pair, through the row sums of the A' integers.
+//
+// Products.  Of the 25 digit pairs, the 15 with i - t >= 1 (weight >= 2^40
+// of a maximum of 2^72) are formed, five int32 accumulators per output
+// (one per i - t); the dropped pairs weigh at most 2^-32 of the largest
+// product each, and since the offsets of all pairs are corrected, they
+// enter as sums of 106 zero-mean terms: about 1e-10 of the result when the
+// bound is a few times the gathered values.  Every int32 sum is exact (at
+// most 5 * 128 * 128 * 128 < 2^24 per accumulator).  The result is
+// assembled in FP64: Y = 2^-(ea+ed) (sum_m 2^(8(4+m)) acc_m + 128 (2^8 +
+// 2^16 + 2^24 + 2^32) sum_q z_q).
+//
+// Reference: the opposite-spin and alpha same-spin terms of
+// lm_apply_array12_same_spin_opt / lm_apply_array12_diff_spin_opt
+// (src/fqe/lib/fqe_data.c:668-720, :879-996), in complex128.
+#include "fqehip_common.h"
+#include "rows_common.h"
+
+namespace fqehip {
+namespace {
+
+using namespace rows;
+
+typedef int int4v __attribute__((ext_vector_type(4)));
+
+constexpr int kOzThreads = 256;            // 4 waves, 16 columns each
+constexpr int kOzCols = 64;                // columns per chunk
+constexpr int kOzKPad = 128;               // gathered rows (pairs + C), padded
+constexpr int kOzKSteps = kOzKPad / 64;    // k steps of the i8 MFMA
+constexpr int kOzADig = 5;                 // digits of A'
+constexpr int kOzDDig = 5;                 // digits 1..5 of the D' integers
+constexpr int kOzARow = kOzKPad + 16;      // bytes per LDS row of a digit plane
+constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52: rint via one add
+
+// rowmax[I] = max over the row of max(|Re C[I][.]|, |Im C[I][.]|).
+__global__ void __launch_bounds__(256)
+k_row_absmax(int64_t lena, int64_t lenb, const double* __restrict__ coeff,
+             double* __restrict__ rowmax, const int32_t* __restrict__ done) {
+  __shared__ double red[4];
+  if (done && *done) return;
+  for (int64_t r = blockIdx.x; r < lena; r += gridDim.x) {
+    const double* row = coeff + r * lenb * 2;
+    double m = 0.0;
+    for (int64_t k = threadIdx.x; k < 2 * lenb; k += 256) m = fmax(m, fabs(row[k]));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      rowmax[r] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    __syncthreads();
+  }
+}
+
+// Digits 1..5 of four 48-bit integers y_j held as (lo, hi) words of
+// kMagic + y_j: out[i - 1] byte j = digit i of y_j (bytes 1..3 of lo, bytes
+// 0..1 of hi), digits 1..4 offset by -128.
+__device__ __forceinline__ void digits_of_four(const uint32_t (&lo)[4],
+                                               const uint32_t (&hi)[4],
+                                               uint32_t (&out)[5]) {
+  out[0] = (((lo[0] >> 8) & 0xffu) | (lo[1] & 0xff00u) |
+            ((lo[2] << 8) & 0xff0000u) | ((lo[3] << 16) & 0xff000000u)) ^ 0x80808080u;
+  out[1] = (((lo[0] >> 16) & 0xffu) | ((lo[1] >> 8) & 0xff00u) |
+            (lo[2] & 0xff0000u) | ((lo[3] << 8) & 0xff000000u)) ^ 0x80808080u;
+  out[2] = ((lo[0] >> 24) | ((lo[1] >> 16) & 0xff00u) |
+            ((lo[2] >> 8) & 0xff0000u) | (lo[3] & 0xff000000u)) ^ 0x80808080u;
+  out[3] = ((hi[0] & 0xffu) | ((hi[1] << 8) & 0xff00u) |
+            ((hi[2] << 16) & 0xff0000u) | (hi[3] << 24)) ^ 0x80808080u;
+  out[4] = ((hi[0] >> 8) & 0xffu) | (hi[1] & 0xff00u) |
+           ((hi[2] << 8) & 0xff0000u) | ((hi[3] << 16) & 0xff000000u);
+}
+
+__device__ __forceinline__ void split_words(double v, uint32_t& lo,
+                                            uint32_t& hi) {
+  const uint64_t b = static_cast<uint64_t>(__double_as_longlong(v));
+  lo = static_cast<uint32_t>(b);
+  hi = static_cast<uint32_t>(b >> 32);
+}
+
+// Alpha rows of the split apply (see k_split_cols_f64 in fused.hip for the
+// operands): per alpha string, the live pair rows' A' digits sit in LDS;
+// each wave owns 16 columns of a 64-column chunk, gathers their D' values
+// (lane l: column l%16, k = 64 s + 16 (l/16) + j), turns them into digit
+// operands in registers and runs the 15 digit products of each of the four
+// 16-row tiles.  i8 MFMA layouts: lane l holds A[row l%16][k 16 (l/16) +
+// j] and B[k 16 (l/16) + j][col l%16] (byte j of its four words); output
+// item e of lane l is [row 4 (l/16) + e][col l%16].
+__global__ void __launch_bounds__(kOzThreads, 2)
+k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
+                int64_t lenb, const double* __restrict__ amat,
+                const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+                const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
+                int nslot, const double* __restrict__ coeff,
+                const double* __restrict__ rowmax, double* __restrict__ slots,
+                double* __restrict__ rowacc, const int32_t* __restrict__ done) {
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int K = npair;
+  const int Kg = K + 1;                     // gathered rows: pairs + C
+  uint8_t* adig = reinterpret_cast<uint8_t*>(smem);  // [5][kLiveRows][kOzARow]
+  double* drow = reinterpret_cast<double*>(adig + kOzADig * kLiveRows * kOzARow);
+  double* rscale = drow + kOzKPad;          // [kLiveRows] 2^-(ea + ed)
+  double* corr = rscale + kLiveRows;        // [kLiveRows] offset correction
+  double* red = corr + kLiveRows;           // [8] block scratch
+  RowTables rt;
+  rt.sbase = reinterpret_cast<int64_t*>(red + 8);
+  rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
+  rt.asgn = rt.aoff + kRowTabs;
+  rt.ssgn = rt.asgn + kRowTabs;
+  int32_t* lpair = rt.ssgn + kRowTabs;      // [kLiveRows] pair of live row
+
+  const int nchunks = static_cast<int>((lenb + kOzCols - 1) / kOzCols);
+  const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(double));
+  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+  const int kq = lane >> 4;                 // k group of the lane
+  const int cl = lane & 15;                 // column within the wave's 16
+
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+    load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
+                            kOzThreads, K);
+    __syncthreads();
+    if (wave == 0) {
+      // compact the live rows: pairs exciting ia to another string, then
+      // the folded row npair
+      int base = 0;
+      for (int p0 = 0; p0 < K; p0 += 64) {
+        const int p = p0 + lane;
+        const bool live = p < K && rt.sbase[p] >= 0;
+        const uint64_t m = __ballot(live);
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (live && pos < kLiveRows) lpair[pos] = p;
+        base += __popcll(m);
+      }
+      if (lane >= base) lpair[lane] = lane == base ? npair : -1;
+    } else if (wave == 1) {
+      // bound on |D'|: alpha sources (sign 1) plus twice the beta part,
+      // which gathers within row ia
+      double m = 0.0;
+      for (int q = lane; q < Kg; q += 64) {
+        const int32_t off = rt.aoff[q];
+        if (off >= 0) m = fmax(m, rowmax[off / (2 * lenb)]);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+      if (lane == 0) red[0] = m + 2.0 * rowmax[ia];
+    }
+    for (int k = tid; k < kOzKPad; k += kOzThreads) {
+      double v = 0.0;
+      if (k < Kg) {
+        for (int i = 0; i < norb; ++i) {
+          const int dp = compressed ? i + i * (i + 1) / 2 : i * (norb + 1);
+          if (rt.sbase[dp] == -2)
+            v += k < K ? amat[static_cast<int64_t>(dp) * K + k]
+                       : amat[static_cast<int64_t>(npair) * K + dp];
+        }
+      }
+      drow[k] = v;
+    }
+    __syncthreads();
+    int ed = 0;
+    {
+      const double bound = red[0];
+      if (bound > 0.0) {
+        int e;
+        (void)frexp(bound, &e);
+        ed = 46 - e;                        // |D' 2^ed| < 2^46
+      }
+    }
+    {
+      // A' digits: thread -> live row tid/4, k in [32 (tid%4), +32)
+      const int m = tid >> 2;
+      const int k0 = (tid & 3) * 32;
+      const int p = lpair[m];
+      double v[32];
+      double amax = 0.0;
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        const int k = k0 + j;
+        double x = 0.0;
+        if (p == npair) {
+          x = drow[k];
+        } else if (p >= 0) {
+          if (k < K) x = amat[static_cast<int64_t>(p) * K + k];
+          else if (k == K) x = amat[static_cast<int64_t>(npair) * K + p];
+        }
+        v[j] = x;
+        amax = fmax(amax, fabs(x));
+      }
+      amax = fmax(amax, __shfl_xor(amax, 1));
+      amax = fmax(amax, __shfl_xor(amax, 2));
+      int ea = 0;
+      if (amax > 0.0) {
+        int e;
+        (void)frexp(amax, &e);
+        ea = 38 - e;                        // |A' 2^ea| < 2^38
+      }
+      // exact balanced digits of the integer z = rint(x 2^ea), |z| < 2^38:
+      // from the low end, d = ((z + 128) mod 256) - 128, z = (z - d) / 256
+      double zsum = 0.0;
+      uint32_t w[kOzADig][8];
+#pragma unroll
+      for (int t = 0; t < kOzADig; ++t)
+#pragma unroll
+        for (int x = 0; x < 8; ++x) w[t][x] = 0u;
+#pragma unroll
+      for (int j = 0; j < 32; ++j) {
+        int64_t z = static_cast<int64_t>(rint(ldexp(v[j], ea)));
+        zsum += static_cast<double>(z);
+#pragma unroll
+        for (int t = kOzADig - 1; t >= 0; --t) {
+          const int64_t d = t ? ((z + 128) & 255) - 128 : z;
+          z = (z - d) >> 8;
+          w[t][j >> 2] |= (static_cast<uint32_t>(d) & 0xffu) << (8 * (j & 3));
+        }
+      }
+      zsum += __shfl_xor(zsum, 1);
+      zsum += __shfl_xor(zsum, 2);
+#pragma unroll
+      for (int t = 0; t < kOzADig; ++t) {
+        uint4* dst = reinterpret_cast<uint4*>(adig + (t * kLiveRows + m) * kOzARow + k0);
+        dst[0] = make_uint4(w[t][0], w[t][1], w[t][2], w[t][3]);
+        dst[1] = make_uint4(w[t][4], w[t][5], w[t][6], w[t][7]);
+      }
+      if ((tid & 3) == 0) {
+        // the -128 offsets of d_1..d_4 over every pair (kept or dropped):
+        // 128 (2^8 + 2^16 + 2^24 + 2^32) sum_q z_q
+        corr[m] = 128.0 * 4311810304.0 * zsum;
+        rscale[m] = ldexp(1.0, -(ea + ed));
+      }
+    }
+    __syncthreads();
+    const double dsc = ldexp(1.0, ed);
+    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(double));
+    double* racc = rowacc + ia * lenb * 2;
+
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
+      const int64_t jb = static_cast<int64_t>(chunk) * kOzCols + wave * 16 + cl;
+      const bool colok = jb < lenb;
+      const int ib = static_cast<int>(colok ? jb : 0);
+      int4v dre[kOzDDig][kOzKSteps], dim[kOzDDig][kOzKSteps];
+#pragma unroll
+      for (int s = 0; s < kOzKSteps; ++s) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          // 8 of the lane's 16 k values of step s
+          int32_t bt[8];
+          double2 ca[8], cb[8];
+          double sa[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int q = 64 * s + 16 * kq + 8 * h + j;
+            bt[j] = bload_i32(rpb, (colok && q < K)
+                                       ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+          }
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int q = 64 * s + 16 * kq + 8 * h + j;
+            const int32_t off = (colok && q < Kg) ? rt.aoff[q] : -1;
+            sa[j] = off >= 0 ? static_cast<double>(rt.asgn[q]) * dsc : 0.0;
+            ca[j] = bload_d2(rc, off >= 0 ? (off + 2 * ib) * 8 : -1);
+            cb[j] = bload_d2(rc, bt[j] ? crow_b + 16 * (iabs(bt[j]) - 1) : -1);
+          }
+#pragma unroll
+          for (int g = 0; g < 2; ++g) {
+            uint32_t lre[4], hre[4], lim[4], him[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int j = 4 * g + u;
+              const double sb = bt[j] > 0 ? 2.0 * dsc : (bt[j] < 0 ? -2.0 * dsc : 0.0);
+              const double vre = fma(sa[j], ca[j].x, fma(sb, cb[j].x, kMagic));
+              const double vim = fma(sa[j], ca[j].y, fma(sb, cb[j].y, kMagic));
+              split_words(vre, lre[u], hre[u]);
+              split_words(vim, lim[u], him[u]);
+            }
+            uint32_t ore[5], oim[5];
+            digits_of_four(lre, hre, ore);
+            digits_of_four(lim, him, oim);
+#pragma unroll
+            for (int i = 0; i < kOzDDig; ++i) {
+              // the digit words exist here, before the next batch's loads
+              // (otherwise the conversions sink to the MFMAs and every
+              // batch's loaded values stay live at once)
+              asm volatile("" : "+v"(ore[i]), "+v"(oim[i]));
+              dre[i][s][2 * h + g] = static_cast<int>(ore[i]);
+              dim[i][s][2 * h + g] = static_cast<int>(oim[i]);
+            }
+          }
+          asm volatile("" ::: "memory");
+        }
+      }
+#pragma unroll 1
+      for (int r = 0; r < kLiveRows / 16; ++r) {
+        int4v accr[5], acci[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          accr[m] = int4v{0, 0, 0, 0};
+          acci[m] = int4v{0, 0, 0, 0};
+        }
+#pragma unroll
+        for (int s = 0; s < kOzKSteps; ++s) {
+          int4v a[kOzADig];
+#pragma unroll
+          for (int t = 0; t < kOzADig; ++t)
+            a[t] = *reinterpret_cast<const int4v*>(
+                adig + (t * kLiveRows + 16 * r + cl) * kOzARow + 64 * s + 16 * kq);
+          // pairs (digit i of D', digit t of A') with i - t = m >= 1
+#pragma unroll
+          for (int i = 1; i <= kOzDDig; ++i)
+#pragma unroll
+            for (int t = 0; t < i; ++t) {
+              accr[i - t - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  a[t], dre[i - 1][s], accr[i - t - 1], 0, 0, 0);
+              acci[i - t - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  a[t], dim[i - 1][s], acci[i - t - 1], 0, 0, 0);
+            }
+        }
+        if (colok) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int row = 16 * r + 4 * kq + e;
+            const int po = lpair[row];
+            if (po < 0) continue;
+            double tr = static_cast<double>(accr[4][e]);
+            double ti = static_cast<double>(acci[4][e]);
+#pragma unroll
+            for (int m = 3; m >= 0; --m) {
+              tr = fma(tr, 256.0, static_cast<double>(accr[m][e]));
+              ti = fma(ti, 256.0, static_cast<double>(acci[m][e]));
+            }
+            const double sc = rscale[row];
+            const double c0 = corr[row];
+            const double yr = fma(tr, 1099511627776.0, c0) * sc;   // 2^40
+            const double yi = fma(ti, 1099511627776.0, c0) * sc;
+            if (po == npair) {
+              double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
+              double2 o = *d;
+              o.x += yr;
+              o.y += yi;
+              *d = o;
+            } else {
+              const int64_t sb = rt.sbase[po];
+              const double sg = static_cast<double>(rt.ssgn[po]);
+              *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
+                  make_double2(sg * yr, sg * yi);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+size_t split_cols_oz_lds_bytes() {
+  return static_cast<size_t>(kOzADig) * kLiveRows * kOzARow +
+         (kOzKPad + 2 * kLiveRows + 8) * sizeof(double) +
+         kRowTabs * (sizeof(int64_t) + 3 * sizeof(int32_t)) +
+         kLiveRows * sizeof(int32_t);
+}
+
+}  // namespace
+
+// Alpha rows of the Ozaki-digit split apply (requirements as for the FP64
+// split kernel: npair + 1 <= 128, norb^2/4 + 1 <= 64 live rows, 32-bit byte
+// offsets); rowmax: the row maxima of |C| (launch_row_absmax first).
+bool split_oz_supported(int norb, int npair, int64_t lena, int64_t lenb) {
+  const int64_t lim = int64_t(1) << 31;
+  return npair + 1 <= kOzKPad && norb * norb / 4 + 1 <= kLiveRows &&
+         static_cast<int64_t>(npair) * lenb * 4 < lim &&
+         lena * lenb * 2 * 8 < lim;
+}
+
+int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
+                    const double* a, const int32_t* pta, const int32_t* ptb,
+                    const int32_t* fwa, const int32_t* sla, int nslot,
+                    const double* coeff, const double* rowmax, double* slots,
+                    double* rowacc, hipStream_t st, const int32_t* done) {
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  if (!split_oz_supported(norb, npair, lena, lenb)) {
+    set_error("the Ozaki split apply needs npair + 1 <= %d and norb^2/4 + 1 <= %d",
+              kOzKPad, kLiveRows);
+    return FQEHIP_ERR_INVALID;
+  }
+  const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
+  const unsigned grid = static_cast<unsigned>(lena < 65536 ? lena : 65536);
+  k_split_cols_oz<<<grid, kOzThreads, split_cols_oz_lds_bytes(), st>>>(
+      norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot, coeff,
+      rowmax, slots, rowacc, done);
+  FQEHIP_CHECK_LAUNCH("k_split_cols_oz");
+  return FQEHIP_OK;
+}
+
+}  // namespace fqehip
+
+// ---------------------------------------------------------------------------
+// Beta-beta half of the Ozaki-digit split apply: rowacc[Ia][Ib'] =
+// sum_Ib C[Ia][Ib] Hbb[Ib'][Ib] as digit products on the int8 matrix cores.
+//
+//   C: each row Ia scaled by 2^ec (|C 2^ec| < 2^46 from the row maximum,
+//      shared by the real and imaginary parts), rounded to 48-bit integers;
+//      digits 1..5 (bits 8..47), 1..4 offset by -128 as in the alpha rows.
+//      M index of the GEMM = 2 Ia + (0 real | 1 imaginary).
+//   Hbb: each column Ib' scaled by 2^eh (|Hbb 2^eh| < 2^38), five balanced
+//      digits, built once per operator (fqehip_beta_block_digits) together
+//      with the column scales and the offset corrections.
+// Products with i - t >= 1, five int32 accumulators per output (exact for
+// lenb < 26214: at most 5 * 128 * 128 * lenb < 2^31).
+// ---------------------------------------------------------------------------
+namespace fqehip {
+namespace {
+
+constexpr int kBgT = 64;                    // tile rows (M) and columns (N)
+constexpr int kBgK = 64;                    // K step
+constexpr int kBgS = kBgK + 16;             // bytes per LDS row (padding)
+constexpr int kBgXcds = 8;                  // XCDs of an MI355X
+constexpr int kBgGroup = 8;                 // row tiles per group of the order
+
+// Tile of workgroup blockIdx.x in an order that keeps each XCD on its own
+// contiguous run of tiles, grouped kBgGroup row tiles at a time (workgroup
+// L runs on XCD L % 8).  False for padding workgroups.
+__device__ __forceinline__ bool oz_tile(int tm, int tn, int per_xcd, int* im,
+                                        int* in) {
+  const int x = static_cast<int>(blockIdx.x % kBgXcds);
+  const int t = x * per_xcd + static_cast<int>(blockIdx.x / kBgXcds);
+  if (t >= tm * tn) return false;
+  const int g = t / (kBgGroup * tn);
+  const int left = tm - g * kBgGroup;
+  const int rows = left < kBgGroup ? left : kBgGroup;
+  const int w = t - g * kBgGroup * tn;
+  *im = g * kBgGroup + w % rows;
+  *in = w / rows;
+  return true;
+}
+
+// C digits: thread -> (row Ia, four consecutive columns); rows of ldk
+// bytes per (digit, M row); cscale[Ia] = 2^-ec.
+__global__ void __launch_bounds__(256)
+k_c_digits(int64_t lena, int64_t lenb, int64_t ldk,
+           const double* __restrict__ coeff, const double* __restrict__ rowmax,
+           uint32_t* __restrict__ cdig, double* __restrict__ cscale,
+           const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t groups = ldk / 4;
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= lena * groups) return;
+  const int64_t ia = idx / groups;
+  const int64_t g = idx - ia * groups;
+  int ec = 0;
+  const double mx = rowmax[ia];
+  if (mx > 0.0) {
+    int e;
+    (void)frexp(mx, &e);
+    ec = 46 - e;
+  }
+  if (g == 0) cscale[ia] = ldexp(1.0, -ec);
+  const double sc = ldexp(1.0, ec);
+  uint32_t lre[4], hre[4], lim[4], him[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t k = 4 * g + u;
+    double xr = 0.0, xi = 0.0;
+    if (k < lenb) {
+      const double2 v = reinterpret_cast<const double2*>(coeff)[ia * lenb + k];
+      xr = v.x;
+      xi = v.y;
+    }
+    split_words(fma(xr, sc, kMagic), lre[u], hre[u]);
+    split_words(fma(xi, sc, kMagic), lim[u], him[u]);
+  }
+  uint32_t ore[5], oim[5];
+  digits_of_four(lre, hre, ore);
+  digits_of_four(lim, him, oim);
+  const int64_t rows = 2 * lena;
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    cdig[((i * rows) + 2 * ia) * groups + g] = ore[i];
+    cdig[((i * rows) + 2 * ia + 1) * groups + g] = oim[i];
+  }
+}
+
+// Hbb column maxima: colmax[Ib'] = max_Ib |Hbb[Ib'][Ib]| (hcol[Ib][Ib']).
+__global__ void __launch_bounds__(256)
+k_hbb_colmax(int64_t lenb, const double* __restrict__ hcol,
+             double* __restrict__ colmax) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (c >= lenb) return;
+  double m = 0.0;
+  for (int64_t k = 0; k < lenb; ++k) m = fmax(m, fabs(hcol[k * lenb + c]));
+  colmax[c] = m;
+}
+
+// Hbb digits: thread -> column Ib'; five exact balanced digits of every K
+// value's integer z (as for A' in the alpha rows), packed four K values per
+// word; then the column's scale 2^-eh and offset correction
+// 128 (2^8 + 2^16 + 2^24 + 2^32) sum_K z.
+__global__ void __launch_bounds__(256)
+k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
+             const double* colmax, uint32_t* __restrict__ hdig,
+             double* __restrict__ hscale, double* hcorr) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (c >= lenb) return;
+  int eh = 0;
+  if (colmax[c] > 0.0) {
+    int e;
+    (void)frexp(colmax[c], &e);
+    eh = 38 - e;
+  }
+  const int64_t groups = ldk / 4;
+  double zsum = 0.0;
+  for (int64_t g = 0; g < groups; ++g) {
+    uint32_t w[5] = {0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = 4 * g + u;
+      int64_t z = k < lenb ? static_cast<int64_t>(rint(ldexp(hcol[k * lenb + c], eh)))
+                           : 0;
+      zsum += static_cast<double>(z);
+#pragma unroll
+      for (int t = 4; t >= 0; --t) {
+        const int64_t d = t ? ((z + 128) & 255) - 128 : z;
+        z = (z - d) >> 8;
+        w[t] |= (static_cast<uint32_t>(d) & 0xffu) << (8 * u);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 5; ++t) hdig[(t * lenb + c) * groups + g] = w[t];
+  }
+  const double corr = 128.0 * 4311810304.0 * zsum;
+  hscale[c] = ldexp(1.0, -eh);
+  hcorr[c] = corr;
+}
+
+// GEMM tile: 64 M rows (32 alpha strings, real and imaginary rows) x 64
+// columns per workgroup of 4 waves (2 x 2, 32 x 32 per wave: 2 x 2 MFMA
+// tiles of 16 x 16, five accumulators each), K in steps of 64 staged
+// through LDS with the next step's global loads in flight.
+__global__ void __launch_bounds__(256, 2)
+k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
+               const uint8_t* __restrict__ cdig,
+               const double* __restrict__ cscale,
+               const uint8_t* __restrict__ hdig,
+               const double* __restrict__ hscale,
+               const double* __restrict__ hcorr, double* __restrict__ out,
+               const int32_t* __restrict__ done, int tm, int tn, int per_xcd) {
+  __shared__ __align__(16) uint8_t sc[5][kBgT * kBgS];
+  __shared__ __align__(16) uint8_t sh[5][kBgT * kBgS];
+  if (done && *done) return;
+  int im, in;
+  if (!oz_tile(tm, tn, per_xcd, &im, &in)) return;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
+  const int64_t rows = 2 * lena;
+  const int64_t m0 = static_cast<int64_t>(im) * kBgT;
+  const int64_t n0 = static_cast<int64_t>(in) * kBgT;
+  const int wm = (wave >> 1) * 32;
+  const int wn = (wave & 1) * 32;
+  const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 5 * rows * ldk);
+  const __amdgpu_buffer_rsrc_t rhd = buffer_of(hdig, 5 * lenb * ldk);
+  // staging: thread -> (tile row tid/4, 16-byte segment tid%4) of each plane
+  const int sr = tid >> 2;
+  const int sseg = 16 * (tid & 3);
+  const bool crow_ok = m0 + sr < rows;
+  const bool hrow_ok = n0 + sr < lenb;
+  u32x4 cv[5], hv[5];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const int oc = crow_ok ? static_cast<int>((i * rows + m0 + sr) * ldk + k0 + sseg) : -1;
+      const int oh = hrow_ok ? static_cast<int>((i * lenb + n0 + sr) * ldk + k0 + sseg) : -1;
+      cv[i] = __builtin_amdgcn_raw_buffer_load_b128(rcd, oc, 0, 0);
+      hv[i] = __builtin_amdgcn_raw_buffer_load_b128(rhd, oh, 0, 0);
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      *reinterpret_cast<u32x4*>(&sc[i][sr * kBgS + sseg]) = cv[i];
+      *reinterpret_cast<u32x4*>(&sh[i][sr * kBgS + sseg]) = hv[i];
+    }
+  };
+  int4v acc[2][2][5];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int m = 0; m < 5; ++m) acc[a][b][m] = int4v{0, 0, 0, 0};
+
+  const int64_t nk = ldk / kBgK;
+  load(0);
+  store();
+  __syncthreads();
+  for (int64_t ks = 0; ks < nk; ++ks) {
+    if (ks + 1 < nk) load((ks + 1) * kBgK);
+    int4v a[2][5], b[2][5];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        a[x][i] = *reinterpret_cast<const int4v*>(
+            &sc[i][(wm + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
+        b[x][i] = *reinterpret_cast<const int4v*>(
+            &sh[i][(wn + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
+      }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int i = 1; i <= 5; ++i)
+#pragma unroll
+          for (int t = 0; t < i; ++t)
+            acc[x][y][i - t - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                a[x][i - 1], b[y][t], acc[x][y][i - t - 1], 0, 0, 0);
+    __syncthreads();
+    if (ks + 1 < nk) {
+      store();
+      __syncthreads();
+    }
+  }
+  // output item e of lane l: M row 4 (l/16) + e, column l%16; rows 2 Ia
+  // (real) and 2 Ia + 1 (imaginary) are items e and e + 1
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int64_t col = n0 + wn + 16 * y + (lane & 15);
+      if (col >= lenb) continue;
+      const double hs = hscale[col];
+      const double hc = hcorr[col];
+#pragma unroll
+      for (int e = 0; e < 4; e += 2) {
+        const int64_t row = m0 + wm + 16 * x + 4 * (lane >> 4) + e;
+        if (row >= rows) continue;
+        const int64_t ia = row >> 1;
+        double v[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          double t = static_cast<double>(acc[x][y][4][e + c]);
+#pragma unroll
+          for (int m = 3; m >= 0; --m)
+            t = fma(t, 256.0, static_cast<double>(acc[x][y][m][e + c]));
+          v[c] = fma(t, 1099511627776.0, hc) * hs;
+        }
+        const double cs = cscale[ia];
+        *reinterpret_cast<double2*>(out + (ia * lenb + col) * 2) =
+            make_double2(v[0] * cs, v[1] * cs);
+      }
+    }
+}
+
+}  // namespace
+
+int64_t beta_oz_ldk(int64_t lenb) { return (lenb + kBgK - 1) / kBgK * kBgK; }
+
+// C digit planes of the apply: 5 x 2 lena x ldk bytes, then lena doubles.
+int64_t beta_oz_workspace_bytes(int64_t lena, int64_t lenb) {
+  return 5 * 2 * lena * beta_oz_ldk(lenb) + 8 * lena;
+}
+
+bool beta_oz_shape_ok(int64_t lena, int64_t lenb) {
+  const int64_t lim = int64_t(1) << 31;
+  return lenb < 26214 && 5 * 2 * lena * beta_oz_ldk(lenb) < lim &&
+         5 * lenb * beta_oz_ldk(lenb) < lim;
+}
+
+int launch_beta_gemm_oz(int64_t lena, int64_t lenb, const void* hoz,
+                        const double* coeff, const double* rowmax,
+                        void* cwork, double* out, hipStream_t st,
+                        const int32_t* done) {
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  const int64_t ldk = beta_oz_ldk(lenb);
+  uint8_t* cdig = static_cast<uint8_t*>(cwork);
+  double* cscale = reinterpret_cast<double*>(cdig + 5 * 2 * lena * ldk);
+  const uint8_t* hdig = static_cast<const uint8_t*>(hoz);
+  const double* hscale = reinterpret_cast<const double*>(hdig + 5 * lenb * ldk);
+  const double* hcorr = hscale + lenb;
+  const int64_t nthr = lena * (ldk / 4);
+  k_c_digits<<<blocks_for(nthr, 256), 256, 0, st>>>(
+      lena, lenb, ldk, coeff, rowmax, reinterpret_cast<uint32_t*>(cdig), cscale,
+      done);
+  FQEHIP_CHECK_LAUNCH("k_c_digits");
+  const int tm = static_cast<int>((2 * lena + kBgT - 1) / kBgT);
+  const int tn = static_cast<int>((lenb + kBgT - 1) / kBgT);
+  const int per_xcd = (tm * tn + kBgXcds - 1) / kBgXcds;
+  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 256, 0, st>>>(
+      lena, lenb, ldk, cdig, cscale, hdig, hscale, hcorr, out, done, tm, tn,
+      per_xcd);
+  FQEHIP_CHECK_LAUNCH("k_beta_gemm_oz");
+  return FQEHIP_OK;
+}
+
+int launch_row_absmax(int64_t lena, int64_t lenb, const double* coeff,
+                      double* rowmax, hipStream_t st, const int32_t* done) {
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  const unsigned grid = static_cast<unsigned>(lena < 65536 ? lena : 65536);
+  k_row_absmax<<<grid, 256, 0, st>>>(lena, lenb, coeff, rowmax, done);
+  FQEHIP_CHECK_LAUNCH("k_row_absmax");
+  return FQEHIP_OK;
+}
+
+}  // namespace fqehip
+
+using namespace fqehip;
+
+extern "C" int64_t fqehip_beta_block_digits_bytes(int64_t lenb) {
+  if (lenb < 0) return -1;
+  return 5 * lenb * beta_oz_ldk(lenb) + 16 * lenb;
+}
+
+extern "C" int fqehip_beta_block_digits(int64_t lenb, const double* d_hcol,
+                                        void* d_hoz, void* stream) {
+  clear_error();
+  FQEHIP_CHECK_ARG(lenb >= 0, "lenb=%lld", static_cast<long long>(lenb));
+  if (lenb == 0) return FQEHIP_OK;
+  FQEHIP_CHECK_ARG(d_hcol && d_hoz, "NULL pointer");
+  FQEHIP_CHECK_ARG(beta_oz_shape_ok(1, lenb), "lenb=%lld too large",
+                   static_cast<long long>(lenb));
+  hipStream_t st = as_stream(stream);
+  const int64_t ldk = beta_oz_ldk(lenb);
+  uint32_t* hdig = static_cast<uint32_t*>(d_hoz);
+  double* hscale = reinterpret_cast<double*>(static_cast<uint8_t*>(d_hoz) + 5 * lenb * ldk);
+  double* hcorr = hscale + lenb;
+  // the correction slots hold the column maxima until the digits pass
+  k_hbb_colmax<<<blocks_for(lenb, 256), 256, 0, st>>>(lenb, d_hcol, hcorr);
+  FQEHIP_CHECK_LAUNCH("k_hbb_colmax");
+  k_hbb_digits<<<blocks_for(lenb, 256), 256, 0, st>>>(lenb, ldk, d_hcol, hcorr,
+                                                      hdig, hscale, hcorr);
+  FQEHIP_CHECK_LAUNCH("k_hbb_digits");
+  return FQEHIP_OK;
+}
+
+extern "C" int fqehip_beta_gemm_digits(int64_t lena, int64_t lenb,
+                                       const void* d_hoz, const void* d_coeff,
+                                       void* d_work, void* d_out,
+                                       const int32_t* d_done, void* stream) {
+  clear_error();
+  FQEHIP_CHECK_ARG(lena >= 0 && lenb >= 0, "negative size");
+  if (lena == 0 || lenb == 0) return FQEHIP_OK;
+  FQEHIP_CHECK_ARG(d_hoz && d_coeff && d_work && d_out, "NULL pointer");
+  FQEHIP_CHECK_ARG(beta_oz_shape_ok(lena, lenb), "sector too large");
+  hipStream_t st = as_stream(stream);
+  double* rowmax = static_cast<double*>(d_work);
+  int rc = launch_row_absmax(lena, lenb, static_cast<const double*>(d_coeff),
+                             rowmax, st, d_done);
+  if (rc) return rc;
+  return launch_beta_gemm_oz(lena, lenb, d_hoz,
+                             static_cast<const double*>(d_coeff), rowmax,
+                             rowmax + lena, static_cast<double*>(d_out), st,
+                             d_done);
+}

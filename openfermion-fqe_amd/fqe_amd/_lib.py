@@ -21,6 +21,8 @@ LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
 C128 = 0
 C64 = 1
 
+TILE_OZAKI = 64   # FQEHIP_TILE_OZAKI
+
 OK = 0
 ERR_INVALID = 1
 ERR_OOM = 2
@@ -39,7 +41,7 @@ class Sector(ctypes.Structure):
                 ("fwd_a", c_void_p), ("fwd_b", c_void_p),
                 ("slot_a", c_void_p), ("nslot", c_int),
                 ("hbb_hi", c_void_p), ("hbb_lo", c_void_p), ("ldh", c_int64),
-                ("tile", c_int), ("hbb_f64", c_void_p)]
+                ("tile", c_int), ("hbb_f64", c_void_p), ("hbb_oz", c_void_p)]
 
 
 class Operator(ctypes.Structure):
@@ -151,15 +153,37 @@ _SIGNATURES = {
                                  c_void_p]),
     "fqehip_beta_gemm_f64": (c_int, [c_int64, c_int64, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p]),
+    "fqehip_beta_block_digits_bytes": (c_int64, [c_int64]),
+    "fqehip_beta_block_digits": (c_int, [c_int64, c_void_p, c_void_p,
+                                         c_void_p]),
+    "fqehip_beta_gemm_digits": (c_int, [c_int64, c_int64, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p,
+                                        c_void_p]),
+    "fqehip_cheb_update": (c_int, [c_int, c_int64, c_void_p, c_void_p,
+                                   c_double, c_double, c_double, c_double,
+                                   c_void_p, c_void_p, c_void_p, c_void_p]),
     "fqehip_apply": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
                              c_void_p, c_double, c_double, c_double, c_double,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_void_p]),
+    "fqehip_apply_panels": (c_int, [c_int, POINTER(Sector),
+                                    POINTER(Operator), c_void_p, c_double,
+                                    c_double, c_double, c_double, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_int64, c_void_p]),
     "fqehip_apply_noalloc": (c_int, [c_int, POINTER(Sector),
                                      POINTER(Operator), c_void_p, c_double,
                                      c_double, c_double, c_double, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p]),
+    "fqehip_taylor_graph_create": (c_int, [c_int, POINTER(Sector),
+                                           POINTER(Operator), c_double,
+                                           c_double, c_int, c_void_p,
+                                           c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_void_p,
+                                           POINTER(c_void_p)]),
+    "fqehip_taylor_graph_launch": (c_int, [c_void_p, c_void_p]),
+    "fqehip_taylor_graph_destroy": (None, [c_void_p]),
     "fqehip_taylor": (c_int, [c_int, POINTER(Sector), POINTER(Operator),
                               c_double, c_double, c_int, c_void_p, c_void_p,
                               c_void_p, c_void_p, c_void_p, c_void_p,

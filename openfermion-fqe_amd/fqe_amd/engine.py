@@ -40,6 +40,7 @@ class PreparedOperator:
         # A[p][q] == A[q][p] exactly (p, q < npair): the split apply applies
         self.pair_symmetric = False
         self._hbb = {}
+        self._hoz = {}
 
     def beta_block(self, graph):
         """Beta-beta block of this operator on the beta strings of
@@ -65,6 +66,24 @@ class PreparedOperator:
                       _lib.ptr(hlo), ldh, _lib.stream_ptr())
             self._hbb[key] = (hhi, hlo, ldh, fwb, hcol)
         return self._hbb[key]
+
+    def beta_block_digits(self, graph):
+        """Ozaki digits of the FP64 beta-beta block (int8 digit planes,
+        column scales and offset corrections; fqehip_beta_block_digits),
+        cached per (nbeta, device); None when the block does not apply."""
+        blk = self.beta_block(graph)
+        if blk is None:
+            return None
+        key = (graph.nbeta(), self.a.device.index)
+        if key not in self._hoz:
+            lenb = graph.lenb()
+            nbytes = int(_lib.call("fqehip_beta_block_digits_bytes", lenb))
+            hoz = torch.empty(max(nbytes, 16), dtype=torch.uint8,
+                              device=self.a.device)
+            _lib.call("fqehip_beta_block_digits", lenb, _lib.ptr(blk[4]),
+                      _lib.ptr(hoz), _lib.stream_ptr())
+            self._hoz[key] = hoz
+        return self._hoz[key]
 
     def c_struct(self) -> _lib.Operator:
         return _lib.Operator(int(self.complex_op), _lib.ptr(self.a),
@@ -164,10 +183,11 @@ def sector_struct(graph, op: PreparedOperator,
     hbb = hbb64 = None
     if fwa is not None and settings.split and precision == _lib.C64:
         hbb = op.beta_block(graph)
-    # FP64 split kernel limits: gather depth npair + 1 <= 112, at most 64
-    # live pair rows (norb^2/4 + 1)
+    # FP64 split kernel limits: gather depth npair + 1 <= 112 (128 for the
+    # digit kernel), at most 64 live pair rows (norb^2/4 + 1)
+    kmax = 128 if settings.contraction == "ozaki" else 112
     if (fwa is not None and settings.split_fp64 and precision == _lib.C128
-            and op.npair + 1 <= 112 and graph.norb() ** 2 // 4 + 1 <= 64):
+            and op.npair + 1 <= kmax and graph.norb() ** 2 // 4 + 1 <= 64):
         blk = op.beta_block(graph)
         hbb64 = blk[4] if blk else None
     s = _lib.Sector(graph.norb(), graph.nalpha(), graph.nbeta(), op.npair,
@@ -182,7 +202,17 @@ def sector_struct(graph, op: PreparedOperator,
     if hbb64 is not None:
         keep.append(hbb64)
         s.hbb_f64 = _lib.ptr(hbb64)
-        s.tile = int(settings.split_tile_fp64)
+        if settings.contraction == "ozaki":
+            s.tile = _lib.TILE_OZAKI
+            if settings.ozaki_beta and graph.lenb() < 26214:
+                hoz = op.beta_block_digits(graph)
+                if hoz is not None:
+                    keep.append(hoz)
+                    s.hbb_oz = _lib.ptr(hoz)
+        elif settings.contraction == "fp64":
+            s.tile = int(settings.split_tile_fp64)
+        else:
+            raise ValueError(f"settings.contraction={settings.contraction!r}")
     return s, tuple(keep)
 
 
@@ -194,11 +224,32 @@ def workspace(sector: _lib.Sector, precision: int,
                                                           _as_ref(sector)))
     elem = 8 if precision == _lib.C128 else 4
     n = sector.lena * sector.lenb
-    need = max(2 * sector.npair + 1, sector.nslot + 1) * 2 * n * elem
+    fused = (sector.nslot + 1) * 2 * n * elem
+    if sector.tile == _lib.TILE_OZAKI and sector.hbb_f64:
+        fused += 8 * sector.lena          # row maxima of |C|
+        if sector.hbb_oz:                 # C digit planes and row scales
+            ldk = (sector.lenb + 63) // 64 * 64
+            fused += 5 * 2 * sector.lena * ldk + 8 * sector.lena
+    need = max((2 * sector.npair + 1) * 2 * n * elem, fused)
     if nbytes != need:
         raise _lib.FqeHipError(f"workspace size {nbytes} != {need}")
     nbytes = max(nbytes, 16 * sector.lena * sector.lenb, 16)
     return torch.empty(nbytes, dtype=torch.uint8, device=device)
+
+
+def needs_panels(sector: _lib.Sector, op: PreparedOperator,
+                 precision: int) -> bool:
+    """True for a complex128 two-body apply on the unfused kernels whose
+    dvec + Y exceed settings.max_workspace_bytes."""
+    if precision != _lib.C128 or op.h1 is not None:
+        return False
+    import ctypes
+    opc = op.c_struct()
+    if _lib.lib().fqehip_fused_supported(precision, ctypes.byref(sector),
+                                         ctypes.byref(opc)):
+        return False
+    n = sector.lena * sector.lenb
+    return (2 * sector.npair + 1) * 2 * n * 8 > settings.max_workspace_bytes
 
 
 def _as_ref(struct):
@@ -213,9 +264,23 @@ def apply_sector(graph, coeff: torch.Tensor, op: PreparedOperator,
                  partials: Optional[torch.Tensor] = None,
                  done: Optional[torch.Tensor] = None,
                  ws: Optional[torch.Tensor] = None) -> None:
-    """out = alpha * H coeff + beta * cin on one sector (device tensors)."""
+    """out = alpha * H coeff + beta * cin on one sector (device tensors).
+    A complex128 sector whose unfused workspace exceeds
+    ``settings.max_workspace_bytes`` runs in panels of orbital pairs."""
     prec = precision_of(coeff.dtype)
     s, keep = sector_struct(graph, op, prec)
+    if ws is None and op.h1 is None and needs_panels(s, op, prec):
+        nbytes = int(settings.max_workspace_bytes)
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=coeff.device)
+        opc = op.c_struct()
+        _lib.call("fqehip_apply_panels", prec, _as_ref(s), _as_ref(opc),
+                  _lib.ptr(coeff), float(numpy.real(alpha)),
+                  float(numpy.imag(alpha)), float(numpy.real(beta)),
+                  float(numpy.imag(beta)), _lib.ptr(cin), _lib.ptr(out),
+                  _lib.ptr(acc), _lib.ptr(partials), _lib.ptr(done),
+                  _lib.ptr(ws), nbytes, _lib.stream_ptr())
+        del keep
+        return
     if ws is None and op.h1 is None:
         ws = workspace(s, prec, coeff.device)
     opc = op.c_struct()

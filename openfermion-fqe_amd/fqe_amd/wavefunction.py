@@ -457,12 +457,75 @@ def _taylor(wfn: Wavefunction, time: float, hamil: RestrictedHamiltonian,
     return out
 
 
+class _TaylorGraph:
+    """One recorded Taylor series (fqehip_taylor_graph_create) with the
+    buffers its nodes reference."""
+
+    def __init__(self, sec: FqeData, op: engine.PreparedOperator,
+                 c: torch.Tensor, time: float, accuracy: float,
+                 expansion: int) -> None:
+        import ctypes
+        dev = c.device
+        self.prec = engine.precision_of(c.dtype)
+        self.s, self.keep = engine.sector_struct(sec.get_fcigraph(), op,
+                                                 self.prec)
+        self.ws = engine.workspace(self.s, self.prec, dev)
+        self.term = torch.empty_like(c)
+        self.result = torch.empty(c.shape, dtype=torch.complex128,
+                                  device=dev)
+        self.partials = torch.zeros(
+            max(engine.norm_partials_count(c.numel()), 1),
+            dtype=torch.float64, device=dev)
+        self.status = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.norm = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.opc = op.c_struct()
+        handle = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        _lib.call("fqehip_taylor_graph_create", self.prec,
+                  engine._as_ref(self.s), engine._as_ref(self.opc),
+                  float(time), float(accuracy), int(expansion),
+                  _lib.ptr(self.term), _lib.ptr(self.result),
+                  _lib.ptr(self.status), _lib.ptr(self.partials),
+                  _lib.ptr(self.norm), _lib.ptr(self.ws),
+                  ctypes.byref(handle))
+        self.handle = handle
+
+    def run(self, c: torch.Tensor):
+        self.term.copy_(c)
+        self.result.copy_(c)
+        _lib.call("fqehip_taylor_graph_launch", self.handle,
+                  _lib.stream_ptr())
+        return self.result.clone(), self.status.cpu().numpy()
+
+    def __del__(self):
+        try:
+            _lib.call("fqehip_taylor_graph_destroy", self.handle)
+        except Exception:
+            pass
+
+
 def _taylor_single(out: Wavefunction, sec: FqeData,
                    op: engine.PreparedOperator, time: float, accuracy: float,
                    expansion: int) -> Wavefunction:
     dev = _lib.device()
     c = sec.device_coeff()
     prec = engine.precision_of(c.dtype)
+    if settings.taylor_graph:
+        g = sec.get_fcigraph()
+        key = (g.nalpha(), g.nbeta(), g.norb(), c.dtype, settings.contraction,
+               settings.split_fp64, settings.split, float(time),
+               float(accuracy), int(expansion))
+        cached = getattr(op, "_taylor_graph", None)
+        if cached is None or cached[0] != key:
+            op._taylor_graph = None          # free the previous one first
+            op._taylor_graph = (key, _TaylorGraph(sec, op, c, time, accuracy,
+                                                  expansion))
+        result, st = op._taylor_graph[1].run(c)
+        if expansion <= 1 or st[1] == 0:
+            raise RuntimeError("maximum taylor expansion limit reached")
+        sec.set_device_coeff(result)
+        out._taylor_terms = int(st[0])
+        return out
     term = c.clone()
     result = c.to(torch.complex128).clone()
     s, keep = engine.sector_struct(sec.get_fcigraph(), op, prec)

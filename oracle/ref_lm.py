@@ -29,7 +29,27 @@ import numpy
 from numpy.ctypeslib import ndpointer
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(_HERE, "_ref", "libfqe_ref.so")
+_V3 = os.path.join(_HERE, "_ref", "libfqe_ref.so")
+_V4 = os.path.join(_HERE, "_ref", "libfqe_ref_v4.so")
+
+
+def _has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq",
+                            "avx512vl"} <= flags
+    except OSError:
+        pass
+    return False
+
+
+# the reference builds with -march=native: the v4 (AVX-512) build where the
+# host has it, else the v3 (AVX2/FMA) one
+LIBPATH = _V4 if os.path.exists(_V4) and _has_avx512() else _V3
+ISA = "x86-64-v4" if LIBPATH == _V4 else "x86-64-v3"
 
 _lib = None
 _blas = None

@@ -99,11 +99,12 @@ def test_adam_matches_oracle(fqe):
     assert numpy.max(numpy.abs(pd.cpu().numpy() - pr)) < 1e-6
 
 
-def test_training_and_gain(fqe):
+@pytest.fixture(scope="module")
+def trained(fqe):
     """Train on the method's own correction vectors of small synthetic
-    Hamiltonians (norb=6, 3a3b) and compare Davidson iterations to a 1e-8
-    residual on held-out ones; the learned run must converge, and its
-    iteration count is reported beside the plain one."""
+    Hamiltonians (norb=6, 3a3b), then solve held-out ones to a 1e-8
+    residual with the plain diagonal preconditioner and with the learned
+    correction."""
     import torch
     from fqe_amd.algorithm import mlp_precond as mp
     xs, ts, ws = [], [], []
@@ -124,20 +125,40 @@ def test_training_and_gain(fqe):
     losses = model.train(x, t, w, epochs=40, batch=1024, lr=2e-3)
     print(f"training: {x.shape[0]} samples, loss {losses[0]:.4e} -> "
           f"{losses[-1]:.4e}", flush=True)
-    assert losses[-1] < losses[0]
-    plain_it, learned_it = [], []
+    runs = []
     for seed in range(6):
         rng = numpy.random.default_rng(5000 + seed)
         h1, h2 = fo.synthetic_hamiltonian(6, 6, rng)
         ham = fqe.get_restricted_hamiltonian((h1, h2))
         r = mp.solve_both(ham, 3, 3, model, tol=1e-8)
-        assert r["plain"].converged and r["learned"].converged
         e0 = numpy.linalg.eigvalsh(
             fo.dense_matrix(h1, h2, fo.OracleGraph(3, 3, 6)))[0]
-        assert abs(r["plain"].eigenvalues[0] - e0) < 1e-9
-        assert abs(r["learned"].eigenvalues[0] - e0) < 1e-9
-        plain_it.append(r["plain"].iterations)
-        learned_it.append(r["learned"].iterations)
+        runs.append((r, e0))
+    plain_it = [r["plain"].iterations for r, _ in runs]
+    learned_it = [r["learned"].iterations for r, _ in runs]
     print(f"held-out iterations plain {plain_it} learned {learned_it} "
           f"(mean {numpy.mean(plain_it):.1f} -> {numpy.mean(learned_it):.1f})",
           flush=True)
+    return losses, runs, plain_it, learned_it
+
+
+def test_training_lowers_loss_and_learned_solves_converge(trained):
+    """The GPU training lowers the loss; with the learned correction the
+    solver still converges to the exact lowest eigenvalue (the residual
+    norm stays the convergence test)."""
+    losses, runs, _, _ = trained
+    assert losses[-1] < losses[0]
+    for r, e0 in runs:
+        assert r["plain"].converged and r["learned"].converged
+        assert abs(r["plain"].eigenvalues[0] - e0) < 1e-9
+        assert abs(r["learned"].eigenvalues[0] - e0) < 1e-9
+
+
+@pytest.mark.xfail(strict=True, reason=(
+    "known shortfall: the learned per-determinant correction does not cut "
+    "Davidson iterations on these Hamiltonians (held-out runs take as many "
+    "or more iterations than the plain diagonal preconditioner); strict, "
+    "so a gain that appears turns this into a failure to update"))
+def test_learned_preconditioner_cuts_iterations_by_30_percent(trained):
+    _, _, plain_it, learned_it = trained
+    assert numpy.mean(learned_it) <= 0.7 * numpy.mean(plain_it)
