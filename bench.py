@@ -389,7 +389,7 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
     beta-beta GEMM is lena x lenb x lenb complex times real (bfloat16
     hi/lo copies of Hbb for the mixed engine, FP64 columns for FP64).
     The Ozaki-digit engine moves the FP64 engine's bytes; its executed
-    int8 operations ("ops") are 15 digit products of the padded tiles: 64
+    int8 operations ("ops") are 20 digit products of the padded tiles: 64
     live rows x 128 gathered rows per column (alpha rows) and 2 lena x
     ldk x lenb (beta-beta GEMM, ldk = lenb rounded up to 64)."""
     from fqe_amd import settings
@@ -404,9 +404,9 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
         lena = graph.lena()
         return {
             "beta_gemm": {"bytes": 2 * 2 * n * esz + 5 * lenb * ldk,
-                          "ops": 15 * 2.0 * (2 * lena) * ldk * lenb},
+                          "ops": 20 * 2.0 * (2 * lena) * ldk * lenb},
             "alpha_rows": {"bytes": (nslot + 3) * 2 * n * esz,
-                           "ops": 15 * 2.0 * 64 * 128 * 2 * n},
+                           "ops": 20 * 2.0 * 64 * 128 * 2 * n},
             "slot_reduce": {"bytes": (nslot + 2) * 2 * n * esz},
         }
     return {
@@ -900,7 +900,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": ("c128 storage; int8 digit products on MFMA (Ozaki: A' 5 "
-                  "digits, D' and C 48-bit integers, 15 digit pairs), exact "
+                  "digits, D' and C 48-bit integers, 20 digit pairs), exact "
                   "int32 accumulation, FP64 assembly"),
         "data": "synthetic",
         "config": {"workload": (f"RestrictedHamiltonian apply, norb={norb} "

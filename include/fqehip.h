@@ -206,10 +206,10 @@ typedef struct fqehip_sector {
 /* complex128 split path (hbb_f64 set): the alpha-row contraction as exact
  * products of 8-bit digits on the int8 matrix cores (Ozaki scheme: A' in
  * five balanced digits per pair row, D' as 48-bit integers per alpha
- * string, 15 digit pairs, int32 accumulation, FP64 assembly); complex128
+ * string, 20 digit pairs, int32 accumulation, FP64 assembly); complex128
  * storage and FP64 slot rows as on the FP64 path.  The workspace grows by
  * lena doubles (row maxima of |C|) and, with hbb_oz set, by the C digit
- * planes of the beta-beta GEMM (5 * 2 lena * ldk bytes + lena doubles). */
+ * planes of the beta-beta GEMM (6 * 2 lena * ldk bytes + lena doubles). */
 #define FQEHIP_TILE_OZAKI 64
 
 typedef struct fqehip_operator {
@@ -262,7 +262,7 @@ int fqehip_beta_block(int npair, int64_t lenb, const int32_t* d_fwd_b,
  * of Hbb, scaled by 2^eh with |Hbb 2^eh| < 2^38 and rounded to integers z,
  * the five exact balanced digits of z, weights 2^32, 2^24, ..., 1), then
  * double [lenb] scales 2^-eh, then double [lenb] offset corrections of
- * the C digits (128 (2^8 + 2^16 + 2^24 + 2^32) sum of the column's z).
+ * the C digits (128 (1 + 2^8 + 2^16 + 2^24 + 2^32) sum of the column's z).
  * Once per operator and beta string count; lenb < 26214. */
 int64_t fqehip_beta_block_digits_bytes(int64_t lenb);
 int fqehip_beta_block_digits(int64_t lenb, const double* d_hcol, void* d_hoz,
@@ -270,7 +270,7 @@ int fqehip_beta_block_digits(int64_t lenb, const double* d_hcol, void* d_hoz,
 /* d_out[Ia][Ib'] = sum_Ib C[Ia][Ib] Hbb[Ib'][Ib] for complex128 C as digit
  * products (the beta-beta half of the Ozaki-digit split apply, for callers
  * that time or schedule it alone): row maxima of |C| into d_work (lena
- * doubles), the C digit planes after them (5 * 2 lena * ldk bytes + lena
+ * doubles), the C digit planes after them (6 * 2 lena * ldk bytes + lena
  * doubles), then the int8 GEMM.  Replaces, like fqehip_beta_gemm_f64, the
  * same-spin beta part of lm_apply_array12_same_spin_opt
  * (src/fqe/lib/fqe_data.c:668-720). */

@@ -38,7 +38,7 @@ int64_t needed_bytes(int precision, const fqehip_sector* s,
     if (precision == FQEHIP_C128 && s->hbb_f64 && s->tile == FQEHIP_TILE_OZAKI) {
       b += s->lena * 8;  // row maxima of |C|
       if (s->hbb_oz)     // C digit planes and row scales
-        b += 5 * 2 * s->lena * ((s->lenb + 63) / 64 * 64) + 8 * s->lena;
+        b += 6 * 2 * s->lena * ((s->lenb + 63) / 64 * 64) + 8 * s->lena;
     }
     return b;
   }

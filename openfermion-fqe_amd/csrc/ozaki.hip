@@ -13,22 +13,23 @@
 // D' (gathered from C) is scaled by one power of two per alpha string,
 // 2^ed with ed from a bound on |D'| (the largest row maximum of |C| over the
 // alpha sources of ia, plus twice the row maximum of row ia), so that
-// |D' 2^ed| < 2^46, and rounded to a 48-bit integer y.  Bytes 1..5 of y
-// (two's complement) are the digits d_1..d_5 of weights 2^8 .. 2^40; the
-// low byte is dropped (2^-38 of the bound).  d_1..d_4 are stored as
-// unsigned bytes minus 128 (one XOR), d_5 as the signed top byte, so no
-// carry propagation is needed; the 128 offsets are added back exactly, for
-// every digit pair, through the row sums of the A' integers.
+// |D' 2^ed| < 2^46, and rounded to a 48-bit integer y.  The six bytes of y
+// (two's complement) are the digits d_0..d_5 of weights 1 .. 2^40.
+// d_0..d_4 are stored as unsigned bytes minus 128 (one XOR), d_5 as the
+// signed top byte, so no carry propagation is needed; the 128 offsets are
+// added back exactly, for every digit pair, through the row sums of the A'
+// integers.
 //
-// Products.  Of the 25 digit pairs, the 15 with i - t >= 1 (weight >= 2^40
-// of a maximum of 2^72) are formed, five int32 accumulators per output
-// (one per i - t); the dropped pairs weigh at most 2^-32 of the largest
-// product each, and since the offsets of all pairs are corrected, they
-// enter as sums of 106 zero-mean terms: about 1e-10 of the result when the
-// bound is a few times the gathered values.  Every int32 sum is exact (at
-// most 5 * 128 * 128 * 128 < 2^24 per accumulator).  The result is
-// assembled in FP64: Y = 2^-(ea+ed) (sum_m 2^(8(4+m)) acc_m + 128 (2^8 +
-// 2^16 + 2^24 + 2^32) sum_q z_q).
+// Products.  Of the 30 digit pairs, the 20 with i - t >= 0 (weight >= 2^32
+// of a maximum of 2^72) are formed, six int32 accumulators per output (one
+// per i - t); the dropped pairs weigh at most 2^-40 of the largest product
+// each, and since the offsets of all pairs are corrected, they enter as
+// sums of zero-mean terms: about 1e-11 of the result when the bound is a
+// few times the gathered values, and still below 1e-9 for vectors whose
+// weight sits on a few determinants.  Every int32 sum is exact (at most
+// 5 * 128 * 128 * 128 < 2^24 per accumulator).  The result is assembled in
+// FP64: Y = 2^-(ea+ed) (2^32 sum_m 2^(8m) acc_m + 128 (1 + 2^8 + 2^16 +
+// 2^24 + 2^32) sum_q z_q).
 //
 // Reference: the opposite-spin and alpha same-spin terms of
 // lm_apply_array12_same_spin_opt / lm_apply_array12_diff_spin_opt
@@ -48,7 +49,7 @@ constexpr int kOzCols = 64;                // columns per chunk
 constexpr int kOzKPad = 128;               // gathered rows (pairs + C), padded
 constexpr int kOzKSteps = kOzKPad / 64;    // k steps of the i8 MFMA
 constexpr int kOzADig = 5;                 // digits of A'
-constexpr int kOzDDig = 5;                 // digits 1..5 of the D' integers
+constexpr int kOzDDig = 6;                 // digits 0..5 of the D' integers
 constexpr int kOzARow = kOzKPad + 16;      // bytes per LDS row of a digit plane
 constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52: rint via one add
 
@@ -72,21 +73,26 @@ k_row_absmax(int64_t lena, int64_t lenb, const double* __restrict__ coeff,
   }
 }
 
-// Digits 1..5 of four 48-bit integers y_j held as (lo, hi) words of
-// kMagic + y_j: out[i - 1] byte j = digit i of y_j (bytes 1..3 of lo, bytes
-// 0..1 of hi), digits 1..4 offset by -128.
+// Digits 0..5 of four 48-bit integers y_j held as the (lo, hi) words of
+// kMagic + y_j: out[i] byte j = digit i of y_j (bytes 0..3 of lo, bytes 0..1
+// of hi); digits 0..4 are unsigned bytes offset by -128 (one XOR), digit 5
+// is the signed top byte.
 __device__ __forceinline__ void digits_of_four(const uint32_t (&lo)[4],
                                                const uint32_t (&hi)[4],
-                                               uint32_t (&out)[5]) {
-  out[0] = (((lo[0] >> 8) & 0xffu) | (lo[1] & 0xff00u) |
+                                               uint32_t (&out)[6]) {
+  // 4 x 4 byte transpose of the low words, then the two bytes of the high
+  // words (the compiler forms v_perm_b32 from these byte selections)
+  out[0] = ((lo[0] & 0xffu) | ((lo[1] & 0xffu) << 8) | ((lo[2] & 0xffu) << 16) |
+            (lo[3] << 24)) ^ 0x80808080u;
+  out[1] = (((lo[0] >> 8) & 0xffu) | (lo[1] & 0xff00u) |
             ((lo[2] << 8) & 0xff0000u) | ((lo[3] << 16) & 0xff000000u)) ^ 0x80808080u;
-  out[1] = (((lo[0] >> 16) & 0xffu) | ((lo[1] >> 8) & 0xff00u) |
+  out[2] = (((lo[0] >> 16) & 0xffu) | ((lo[1] >> 8) & 0xff00u) |
             (lo[2] & 0xff0000u) | ((lo[3] << 8) & 0xff000000u)) ^ 0x80808080u;
-  out[2] = ((lo[0] >> 24) | ((lo[1] >> 16) & 0xff00u) |
+  out[3] = ((lo[0] >> 24) | ((lo[1] >> 16) & 0xff00u) |
             ((lo[2] >> 8) & 0xff0000u) | (lo[3] & 0xff000000u)) ^ 0x80808080u;
-  out[3] = ((hi[0] & 0xffu) | ((hi[1] << 8) & 0xff00u) |
+  out[4] = ((hi[0] & 0xffu) | ((hi[1] << 8) & 0xff00u) |
             ((hi[2] << 16) & 0xff0000u) | (hi[3] << 24)) ^ 0x80808080u;
-  out[4] = ((hi[0] >> 8) & 0xffu) | (hi[1] & 0xff00u) |
+  out[5] = ((hi[0] >> 8) & 0xffu) | (hi[1] & 0xff00u) |
            ((hi[2] << 8) & 0xff0000u) | ((hi[3] << 16) & 0xff000000u);
 }
 
@@ -97,14 +103,33 @@ __device__ __forceinline__ void split_words(double v, uint32_t& lo,
   hi = static_cast<uint32_t>(b >> 32);
 }
 
+// Offset correction constant: the -128 offsets of data digits 0..4 over
+// every digit pair, 128 (1 + 2^8 + 2^16 + 2^24 + 2^32) per unit of the
+// operator integers' sum.
+constexpr double kOffsetUnit = 128.0 * 4311810305.0;
+// Invalid gather offset: past the end of every C buffer the kernel reads
+// (lena * lenb * 16 < kNoRow is a launch requirement) and still a valid
+// int32 after adding a column offset (16 * lenb < 2^20).
+constexpr int32_t kNoRow = 0x7ff00000;
+
+// Per-q entry of the alpha gather table (LDS, per alpha string): byte
+// offset of the source row of C (kNoRow: none) and the sign of the entry
+// times the data scale 2^ed.
+struct GatherEntry {
+  int32_t off;
+  int32_t pad;
+  double sa;
+};
+
 // Alpha rows of the split apply (see k_split_cols_f64 in fused.hip for the
 // operands): per alpha string, the live pair rows' A' digits sit in LDS;
 // each wave owns 16 columns of a 64-column chunk, gathers their D' values
-// (lane l: column l%16, k = 64 s + 16 (l/16) + j), turns them into digit
-// operands in registers and runs the 15 digit products of each of the four
-// 16-row tiles.  i8 MFMA layouts: lane l holds A[row l%16][k 16 (l/16) +
-// j] and B[k 16 (l/16) + j][col l%16] (byte j of its four words); output
-// item e of lane l is [row 4 (l/16) + e][col l%16].
+// (lane l: column l%16, k = 64 s + 16 (l/16) + j), turns them into six
+// digit operands in registers and runs the 20 digit products (weight
+// classes m = i - t = 0..5) of each of the four 16-row tiles, the real and
+// imaginary parts one after the other.  i8 MFMA layouts: lane l holds
+// A[row l%16][k 16 (l/16) + j] and B[k 16 (l/16) + j][col l%16] (byte j of
+// its four words); output item e of lane l is [row 4 (l/16) + e][col l%16].
 __global__ void __launch_bounds__(kOzThreads, 2)
 k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
                 int64_t lenb, const double* __restrict__ amat,
@@ -121,7 +146,9 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
   const int K = npair;
   const int Kg = K + 1;                     // gathered rows: pairs + C
   uint8_t* adig = reinterpret_cast<uint8_t*>(smem);  // [5][kLiveRows][kOzARow]
-  double* drow = reinterpret_cast<double*>(adig + kOzADig * kLiveRows * kOzARow);
+  GatherEntry* gtab = reinterpret_cast<GatherEntry*>(
+      adig + kOzADig * kLiveRows * kOzARow);          // [kOzKPad]
+  double* drow = reinterpret_cast<double*>(gtab + kOzKPad);  // [kOzKPad]
   double* rscale = drow + kOzKPad;          // [kLiveRows] 2^-(ea + ed)
   double* corr = rscale + kLiveRows;        // [kLiveRows] offset correction
   double* red = corr + kLiveRows;           // [8] block scratch
@@ -137,6 +164,7 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
   const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
   const int kq = lane >> 4;                 // k group of the lane
   const int cl = lane & 15;                 // column within the wave's 16
+  const int lenb4 = static_cast<int>(lenb) * 4;
 
   for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
     load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
@@ -188,6 +216,15 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
         (void)frexp(bound, &e);
         ed = 46 - e;                        // |D' 2^ed| < 2^46
       }
+    }
+    const double dsc = ldexp(1.0, ed);
+    if (tid < kOzKPad) {
+      const int32_t off = tid < Kg ? rt.aoff[tid] : -1;
+      GatherEntry g;
+      g.off = off >= 0 ? off * 8 : kNoRow;
+      g.pad = 0;
+      g.sa = off >= 0 ? static_cast<double>(rt.asgn[tid]) * dsc : 0.0;
+      gtab[tid] = g;
     }
     {
       // A' digits: thread -> live row tid/4, k in [32 (tid%4), +32)
@@ -245,20 +282,19 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
         dst[1] = make_uint4(w[t][4], w[t][5], w[t][6], w[t][7]);
       }
       if ((tid & 3) == 0) {
-        // the -128 offsets of d_1..d_4 over every pair (kept or dropped):
-        // 128 (2^8 + 2^16 + 2^24 + 2^32) sum_q z_q
-        corr[m] = 128.0 * 4311810304.0 * zsum;
+        corr[m] = kOffsetUnit * zsum;
         rscale[m] = ldexp(1.0, -(ea + ed));
       }
     }
     __syncthreads();
-    const double dsc = ldexp(1.0, ed);
     const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(double));
     double* racc = rowacc + ia * lenb * 2;
+    const double dsc2 = 2.0 * dsc;
 
     for (int chunk = 0; chunk < nchunks; ++chunk) {
       const int64_t jb = static_cast<int64_t>(chunk) * kOzCols + wave * 16 + cl;
       const bool colok = jb < lenb;
+      // padding columns gather column 0 (valid memory) and store nothing
       const int ib = static_cast<int>(colok ? jb : 0);
       int4v dre[kOzDDig][kOzKSteps], dim[kOzDDig][kOzKSteps];
 #pragma unroll
@@ -266,22 +302,21 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           // 8 of the lane's 16 k values of step s
+          const int q0 = 64 * s + 16 * kq + 8 * h;
           int32_t bt[8];
           double2 ca[8], cb[8];
-          double sa[8];
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const int q = 64 * s + 16 * kq + 8 * h + j;
-            bt[j] = bload_i32(rpb, (colok && q < K)
-                                       ? (q * static_cast<int>(lenb) + ib) * 4 : -1);
+            const int q = q0 + j;
+            bt[j] = bload_i32(rpb, q < K ? q * lenb4 + 4 * ib : -1);
           }
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const int q = 64 * s + 16 * kq + 8 * h + j;
-            const int32_t off = (colok && q < Kg) ? rt.aoff[q] : -1;
-            sa[j] = off >= 0 ? static_cast<double>(rt.asgn[q]) * dsc : 0.0;
-            ca[j] = bload_d2(rc, off >= 0 ? (off + 2 * ib) * 8 : -1);
-            cb[j] = bload_d2(rc, bt[j] ? crow_b + 16 * (iabs(bt[j]) - 1) : -1);
+            const GatherEntry g = gtab[q0 + j];
+            ca[j] = bload_d2(rc, g.off + 16 * ib);
+            // bt == 0 reads a neighbouring value that its zero weight drops
+            const int32_t ab = bt[j] < 0 ? -bt[j] : bt[j];
+            cb[j] = bload_d2(rc, crow_b + 16 * ab - 16);
           }
 #pragma unroll
           for (int g = 0; g < 2; ++g) {
@@ -289,13 +324,15 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const int j = 4 * g + u;
-              const double sb = bt[j] > 0 ? 2.0 * dsc : (bt[j] < 0 ? -2.0 * dsc : 0.0);
-              const double vre = fma(sa[j], ca[j].x, fma(sb, cb[j].x, kMagic));
-              const double vim = fma(sa[j], ca[j].y, fma(sb, cb[j].y, kMagic));
+              const double sa = gtab[q0 + j].sa;
+              const int32_t sgn = bt[j] > 0 ? 1 : (bt[j] < 0 ? -1 : 0);
+              const double sb = static_cast<double>(sgn) * dsc2;
+              const double vre = fma(sa, ca[j].x, fma(sb, cb[j].x, kMagic));
+              const double vim = fma(sa, ca[j].y, fma(sb, cb[j].y, kMagic));
               split_words(vre, lre[u], hre[u]);
               split_words(vim, lim[u], him[u]);
             }
-            uint32_t ore[5], oim[5];
+            uint32_t ore[kOzDDig], oim[kOzDDig];
             digits_of_four(lre, hre, ore);
             digits_of_four(lim, him, oim);
 #pragma unroll
@@ -313,58 +350,54 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
       }
 #pragma unroll 1
       for (int r = 0; r < kLiveRows / 16; ++r) {
-        int4v accr[5], acci[5];
+        double yre[4];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-          accr[m] = int4v{0, 0, 0, 0};
-          acci[m] = int4v{0, 0, 0, 0};
-        }
+        for (int c = 0; c < 2; ++c) {
+          int4v acc[6];
 #pragma unroll
-        for (int s = 0; s < kOzKSteps; ++s) {
-          int4v a[kOzADig];
+          for (int m = 0; m < 6; ++m) acc[m] = int4v{0, 0, 0, 0};
 #pragma unroll
-          for (int t = 0; t < kOzADig; ++t)
-            a[t] = *reinterpret_cast<const int4v*>(
-                adig + (t * kLiveRows + 16 * r + cl) * kOzARow + 64 * s + 16 * kq);
-          // pairs (digit i of D', digit t of A') with i - t = m >= 1
+          for (int s = 0; s < kOzKSteps; ++s) {
+            int4v a[kOzADig];
 #pragma unroll
-          for (int i = 1; i <= kOzDDig; ++i)
+            for (int t = 0; t < kOzADig; ++t)
+              a[t] = *reinterpret_cast<const int4v*>(
+                  adig + (t * kLiveRows + 16 * r + cl) * kOzARow + 64 * s + 16 * kq);
+            // pairs (data digit i, operator digit t) with m = i - t >= 0
 #pragma unroll
-            for (int t = 0; t < i; ++t) {
-              accr[i - t - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                  a[t], dre[i - 1][s], accr[i - t - 1], 0, 0, 0);
-              acci[i - t - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                  a[t], dim[i - 1][s], acci[i - t - 1], 0, 0, 0);
-            }
-        }
-        if (colok) {
+            for (int i = 0; i < kOzDDig; ++i)
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int row = 16 * r + 4 * kq + e;
-            const int po = lpair[row];
-            if (po < 0) continue;
-            double tr = static_cast<double>(accr[4][e]);
-            double ti = static_cast<double>(acci[4][e]);
+              for (int t = 0; t < kOzADig; ++t)
+                if (t <= i)
+                  acc[i - t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                      a[t], c ? dim[i][s] : dre[i][s], acc[i - t], 0, 0, 0);
+          }
+          if (colok) {
 #pragma unroll
-            for (int m = 3; m >= 0; --m) {
-              tr = fma(tr, 256.0, static_cast<double>(accr[m][e]));
-              ti = fma(ti, 256.0, static_cast<double>(acci[m][e]));
-            }
-            const double sc = rscale[row];
-            const double c0 = corr[row];
-            const double yr = fma(tr, 1099511627776.0, c0) * sc;   // 2^40
-            const double yi = fma(ti, 1099511627776.0, c0) * sc;
-            if (po == npair) {
-              double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
-              double2 o = *d;
-              o.x += yr;
-              o.y += yi;
-              *d = o;
-            } else {
-              const int64_t sb = rt.sbase[po];
-              const double sg = static_cast<double>(rt.ssgn[po]);
-              *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
-                  make_double2(sg * yr, sg * yi);
+            for (int e = 0; e < 4; ++e) {
+              const int row = 16 * r + 4 * kq + e;
+              double t = static_cast<double>(acc[5][e]);
+#pragma unroll
+              for (int m = 4; m >= 0; --m)
+                t = fma(t, 256.0, static_cast<double>(acc[m][e]));
+              const double y = fma(t, 4294967296.0, corr[row]) * rscale[row];
+              if (c == 0) {
+                yre[e] = y;
+              } else {
+                const int po = lpair[row];
+                if (po == npair) {
+                  double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
+                  double2 o = *d;
+                  o.x += yre[e];
+                  o.y += y;
+                  *d = o;
+                } else if (po >= 0) {
+                  const int64_t sb = rt.sbase[po];
+                  const double sg = static_cast<double>(rt.ssgn[po]);
+                  *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
+                      make_double2(sg * yre[e], sg * y);
+                }
+              }
             }
           }
         }
@@ -376,6 +409,7 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
 
 size_t split_cols_oz_lds_bytes() {
   return static_cast<size_t>(kOzADig) * kLiveRows * kOzARow +
+         kOzKPad * sizeof(GatherEntry) +
          (kOzKPad + 2 * kLiveRows + 8) * sizeof(double) +
          kRowTabs * (sizeof(int64_t) + 3 * sizeof(int32_t)) +
          kLiveRows * sizeof(int32_t);
@@ -390,7 +424,7 @@ bool split_oz_supported(int norb, int npair, int64_t lena, int64_t lenb) {
   const int64_t lim = int64_t(1) << 31;
   return npair + 1 <= kOzKPad && norb * norb / 4 + 1 <= kLiveRows &&
          static_cast<int64_t>(npair) * lenb * 4 < lim &&
-         lena * lenb * 2 * 8 < lim;
+         lena * lenb * 2 * 8 < kNoRow && lenb < (1 << 16);
 }
 
 int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
@@ -421,13 +455,13 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
 //
 //   C: each row Ia scaled by 2^ec (|C 2^ec| < 2^46 from the row maximum,
 //      shared by the real and imaginary parts), rounded to 48-bit integers;
-//      digits 1..5 (bits 8..47), 1..4 offset by -128 as in the alpha rows.
+//      digits 0..5 (bytes), 0..4 offset by -128 as in the alpha rows.
 //      M index of the GEMM = 2 Ia + (0 real | 1 imaginary).
 //   Hbb: each column Ib' scaled by 2^eh (|Hbb 2^eh| < 2^38), five balanced
 //      digits, built once per operator (fqehip_beta_block_digits) together
 //      with the column scales and the offset corrections.
-// Products with i - t >= 1, five int32 accumulators per output (exact for
-// lenb < 26214: at most 5 * 128 * 128 * lenb < 2^31).
+// Products with i - t >= 0 (20 pairs), six int32 accumulators per output
+// (exact for lenb < 26214: at most 5 * 128 * 128 * lenb < 2^31).
 // ---------------------------------------------------------------------------
 namespace fqehip {
 namespace {
@@ -490,12 +524,12 @@ k_c_digits(int64_t lena, int64_t lenb, int64_t ldk,
     split_words(fma(xr, sc, kMagic), lre[u], hre[u]);
     split_words(fma(xi, sc, kMagic), lim[u], him[u]);
   }
-  uint32_t ore[5], oim[5];
+  uint32_t ore[6], oim[6];
   digits_of_four(lre, hre, ore);
   digits_of_four(lim, him, oim);
   const int64_t rows = 2 * lena;
 #pragma unroll
-  for (int i = 0; i < 5; ++i) {
+  for (int i = 0; i < 6; ++i) {
     cdig[((i * rows) + 2 * ia) * groups + g] = ore[i];
     cdig[((i * rows) + 2 * ia + 1) * groups + g] = oim[i];
   }
@@ -515,7 +549,7 @@ k_hbb_colmax(int64_t lenb, const double* __restrict__ hcol,
 // Hbb digits: thread -> column Ib'; five exact balanced digits of every K
 // value's integer z (as for A' in the alpha rows), packed four K values per
 // word; then the column's scale 2^-eh and offset correction
-// 128 (2^8 + 2^16 + 2^24 + 2^32) sum_K z.
+// 128 (1 + 2^8 + 2^16 + 2^24 + 2^32) sum_K z.
 __global__ void __launch_bounds__(256)
 k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
              const double* colmax, uint32_t* __restrict__ hdig,
@@ -548,14 +582,14 @@ k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
 #pragma unroll
     for (int t = 0; t < 5; ++t) hdig[(t * lenb + c) * groups + g] = w[t];
   }
-  const double corr = 128.0 * 4311810304.0 * zsum;
+  const double corr = kOffsetUnit * zsum;
   hscale[c] = ldexp(1.0, -eh);
   hcorr[c] = corr;
 }
 
 // GEMM tile: 64 M rows (32 alpha strings, real and imaginary rows) x 64
 // columns per workgroup of 4 waves (2 x 2, 32 x 32 per wave: 2 x 2 MFMA
-// tiles of 16 x 16, five accumulators each), K in steps of 64 staged
+// tiles of 16 x 16, six accumulators each), K in steps of 64 staged
 // through LDS with the next step's global loads in flight.
 __global__ void __launch_bounds__(256, 2)
 k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
@@ -565,7 +599,7 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
                const double* __restrict__ hscale,
                const double* __restrict__ hcorr, double* __restrict__ out,
                const int32_t* __restrict__ done, int tm, int tn, int per_xcd) {
-  __shared__ __align__(16) uint8_t sc[5][kBgT * kBgS];
+  __shared__ __align__(16) uint8_t sc[6][kBgT * kBgS];
   __shared__ __align__(16) uint8_t sh[5][kBgT * kBgS];
   if (done && *done) return;
   int im, in;
@@ -578,37 +612,41 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   const int64_t n0 = static_cast<int64_t>(in) * kBgT;
   const int wm = (wave >> 1) * 32;
   const int wn = (wave & 1) * 32;
-  const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 5 * rows * ldk);
+  const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 6 * rows * ldk);
   const __amdgpu_buffer_rsrc_t rhd = buffer_of(hdig, 5 * lenb * ldk);
   // staging: thread -> (tile row tid/4, 16-byte segment tid%4) of each plane
   const int sr = tid >> 2;
   const int sseg = 16 * (tid & 3);
   const bool crow_ok = m0 + sr < rows;
   const bool hrow_ok = n0 + sr < lenb;
-  u32x4 cv[5], hv[5];
+  u32x4 cv[6], hv[5];
   auto load = [&](int64_t k0) {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 6; ++i) {
       const int oc = crow_ok ? static_cast<int>((i * rows + m0 + sr) * ldk + k0 + sseg) : -1;
-      const int oh = hrow_ok ? static_cast<int>((i * lenb + n0 + sr) * ldk + k0 + sseg) : -1;
       cv[i] = __builtin_amdgcn_raw_buffer_load_b128(rcd, oc, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      const int oh = hrow_ok ? static_cast<int>((i * lenb + n0 + sr) * ldk + k0 + sseg) : -1;
       hv[i] = __builtin_amdgcn_raw_buffer_load_b128(rhd, oh, 0, 0);
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 6; ++i)
       *reinterpret_cast<u32x4*>(&sc[i][sr * kBgS + sseg]) = cv[i];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
       *reinterpret_cast<u32x4*>(&sh[i][sr * kBgS + sseg]) = hv[i];
-    }
   };
-  int4v acc[2][2][5];
+  int4v acc[2][2][6];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
-      for (int m = 0; m < 5; ++m) acc[a][b][m] = int4v{0, 0, 0, 0};
+      for (int m = 0; m < 6; ++m) acc[a][b][m] = int4v{0, 0, 0, 0};
 
   const int64_t nk = ldk / kBgK;
   load(0);
@@ -616,26 +654,30 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   __syncthreads();
   for (int64_t ks = 0; ks < nk; ++ks) {
     if (ks + 1 < nk) load((ks + 1) * kBgK);
-    int4v a[2][5], b[2][5];
+    int4v a[2][6];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-      for (int i = 0; i < 5; ++i) {
+      for (int i = 0; i < 6; ++i)
         a[x][i] = *reinterpret_cast<const int4v*>(
             &sc[i][(wm + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
-        b[x][i] = *reinterpret_cast<const int4v*>(
-            &sh[i][(wn + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
-      }
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) {
+      int4v b[5];
 #pragma unroll
-      for (int y = 0; y < 2; ++y)
+      for (int i = 0; i < 5; ++i)
+        b[i] = *reinterpret_cast<const int4v*>(
+            &sh[i][(wn + 16 * y + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
 #pragma unroll
-        for (int i = 1; i <= 5; ++i)
+      for (int x = 0; x < 2; ++x)
 #pragma unroll
-          for (int t = 0; t < i; ++t)
-            acc[x][y][i - t - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                a[x][i - 1], b[y][t], acc[x][y][i - t - 1], 0, 0, 0);
+        for (int i = 0; i < 6; ++i)
+#pragma unroll
+          for (int t = 0; t < 5; ++t)
+            if (t <= i)
+              acc[x][y][i - t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  a[x][i], b[t], acc[x][y][i - t], 0, 0, 0);
+    }
     __syncthreads();
     if (ks + 1 < nk) {
       store();
@@ -660,11 +702,11 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
         double v[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-          double t = static_cast<double>(acc[x][y][4][e + c]);
+          double t = static_cast<double>(acc[x][y][5][e + c]);
 #pragma unroll
-          for (int m = 3; m >= 0; --m)
+          for (int m = 4; m >= 0; --m)
             t = fma(t, 256.0, static_cast<double>(acc[x][y][m][e + c]));
-          v[c] = fma(t, 1099511627776.0, hc) * hs;
+          v[c] = fma(t, 4294967296.0, hc) * hs;
         }
         const double cs = cscale[ia];
         *reinterpret_cast<double2*>(out + (ia * lenb + col) * 2) =
@@ -677,14 +719,14 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
 
 int64_t beta_oz_ldk(int64_t lenb) { return (lenb + kBgK - 1) / kBgK * kBgK; }
 
-// C digit planes of the apply: 5 x 2 lena x ldk bytes, then lena doubles.
+// C digit planes of the apply: 6 x 2 lena x ldk bytes, then lena doubles.
 int64_t beta_oz_workspace_bytes(int64_t lena, int64_t lenb) {
-  return 5 * 2 * lena * beta_oz_ldk(lenb) + 8 * lena;
+  return 6 * 2 * lena * beta_oz_ldk(lenb) + 8 * lena;
 }
 
 bool beta_oz_shape_ok(int64_t lena, int64_t lenb) {
   const int64_t lim = int64_t(1) << 31;
-  return lenb < 26214 && 5 * 2 * lena * beta_oz_ldk(lenb) < lim &&
+  return lenb < 26214 && 6 * 2 * lena * beta_oz_ldk(lenb) < lim &&
          5 * lenb * beta_oz_ldk(lenb) < lim;
 }
 
@@ -695,7 +737,7 @@ int launch_beta_gemm_oz(int64_t lena, int64_t lenb, const void* hoz,
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   const int64_t ldk = beta_oz_ldk(lenb);
   uint8_t* cdig = static_cast<uint8_t*>(cwork);
-  double* cscale = reinterpret_cast<double*>(cdig + 5 * 2 * lena * ldk);
+  double* cscale = reinterpret_cast<double*>(cdig + 6 * 2 * lena * ldk);
   const uint8_t* hdig = static_cast<const uint8_t*>(hoz);
   const double* hscale = reinterpret_cast<const double*>(hdig + 5 * lenb * ldk);
   const double* hcorr = hscale + lenb;

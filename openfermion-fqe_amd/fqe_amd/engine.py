@@ -229,7 +229,7 @@ def workspace(sector: _lib.Sector, precision: int,
         fused += 8 * sector.lena          # row maxima of |C|
         if sector.hbb_oz:                 # C digit planes and row scales
             ldk = (sector.lenb + 63) // 64 * 64
-            fused += 5 * 2 * sector.lena * ldk + 8 * sector.lena
+            fused += 6 * 2 * sector.lena * ldk + 8 * sector.lena
     need = max((2 * sector.npair + 1) * 2 * n * elem, fused)
     if nbytes != need:
         raise _lib.FqeHipError(f"workspace size {nbytes} != {need}")
