@@ -461,11 +461,13 @@ int launch_fused(int precision, int norb, int npair, int64_t lena, int64_t lenb,
                  bool split, int tile, hipStream_t st, const int32_t* done);
 // Defined in ozaki.hip.
 bool split_oz_supported(int norb, int npair, int64_t lena, int64_t lenb);
+int64_t split_oz_table_bytes(int64_t lenb);
 int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
                     const int32_t* fwa, const int32_t* sla, int nslot,
-                    const double* coeff, const double* rowmax, double* slots,
-                    double* rowacc, hipStream_t st, const int32_t* done);
+                    const double* coeff, const double* rowmax, void* btab,
+                    double* slots, double* rowacc, hipStream_t st,
+                    const int32_t* done);
 int launch_row_absmax(int64_t lena, int64_t lenb, const double* coeff,
                       double* rowmax, hipStream_t st, const int32_t* done);
 bool beta_oz_shape_ok(int64_t lena, int64_t lenb);
@@ -685,12 +687,15 @@ bool ozaki_split(int precision, const fqehip_sector* s) {
 }
 
 // Workspace of the fused / split paths: slot rows, row accumulators and,
-// for the Ozaki-digit split apply, the row maxima of |C| (lena doubles).
+// for the Ozaki-digit split apply, the beta gather table (lenb x 128
+// words), the row maxima of |C| (lena doubles) and, with hbb_oz, the C
+// digit planes of the beta-beta GEMM.
 int64_t fused_workspace_bytes(int precision, const fqehip_sector* s) {
   const int64_t n = s->lena * s->lenb;
   int64_t b = static_cast<int64_t>(s->nslot + 1) * 2 * n *
               static_cast<int64_t>(elem_size(precision));
   if (ozaki_split(precision, s)) {
+    b += split_oz_table_bytes(s->lenb);
     b += s->lena * static_cast<int64_t>(sizeof(double));  // row maxima
     if (s->hbb_oz) b += beta_oz_workspace_bytes(s->lena, s->lenb);
   }
@@ -733,7 +738,9 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
     // row maxima of |C| (digit scales), rowacc <- C Hbb^T (digit products
     // when the operator's Hbb digits are set, else FP64), then the alpha
     // rows (digit products)
-    double* rowmax = static_cast<double*>(rowacc) + 2 * n;
+    void* btab = static_cast<double*>(rowacc) + 2 * n;
+    double* rowmax = reinterpret_cast<double*>(static_cast<char*>(btab) +
+                                               split_oz_table_bytes(s->lenb));
     int rc = launch_row_absmax(s->lena, s->lenb,
                                static_cast<const double*>(d_coeff), rowmax, st,
                                d_done);
@@ -750,7 +757,7 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
     if (rc) return rc;
     return launch_split_oz(s->norb, s->npair, s->lena, s->lenb, op->a,
                            s->ptab_a, s->ptab_b, s->fwd_a, s->slot_a, s->nslot,
-                           static_cast<const double*>(d_coeff), rowmax,
+                           static_cast<const double*>(d_coeff), rowmax, btab,
                            reinterpret_cast<double*>(ws),
                            static_cast<double*>(rowacc), st, d_done);
   }

@@ -44,8 +44,6 @@ using namespace rows;
 
 typedef int int4v __attribute__((ext_vector_type(4)));
 
-constexpr int kOzThreads = 256;            // 4 waves, 16 columns each
-constexpr int kOzCols = 64;                // columns per chunk
 constexpr int kOzKPad = 128;               // gathered rows (pairs + C), padded
 constexpr int kOzKSteps = kOzKPad / 64;    // k steps of the i8 MFMA
 constexpr int kOzADig = 5;                 // digits of A'
@@ -76,24 +74,24 @@ k_row_absmax(int64_t lena, int64_t lenb, const double* __restrict__ coeff,
 // Digits 0..5 of four 48-bit integers y_j held as the (lo, hi) words of
 // kMagic + y_j: out[i] byte j = digit i of y_j (bytes 0..3 of lo, bytes 0..1
 // of hi); digits 0..4 are unsigned bytes offset by -128 (one XOR), digit 5
-// is the signed top byte.
+// is the signed top byte.  A two-stage byte transpose with v_perm_b32
+// (perm(a, b, sel): result byte k = byte sel_k of the 8 bytes b0..b3 a0..a3).
 __device__ __forceinline__ void digits_of_four(const uint32_t (&lo)[4],
                                                const uint32_t (&hi)[4],
                                                uint32_t (&out)[6]) {
-  // 4 x 4 byte transpose of the low words, then the two bytes of the high
-  // words (the compiler forms v_perm_b32 from these byte selections)
-  out[0] = ((lo[0] & 0xffu) | ((lo[1] & 0xffu) << 8) | ((lo[2] & 0xffu) << 16) |
-            (lo[3] << 24)) ^ 0x80808080u;
-  out[1] = (((lo[0] >> 8) & 0xffu) | (lo[1] & 0xff00u) |
-            ((lo[2] << 8) & 0xff0000u) | ((lo[3] << 16) & 0xff000000u)) ^ 0x80808080u;
-  out[2] = (((lo[0] >> 16) & 0xffu) | ((lo[1] >> 8) & 0xff00u) |
-            (lo[2] & 0xff0000u) | ((lo[3] << 8) & 0xff000000u)) ^ 0x80808080u;
-  out[3] = ((lo[0] >> 24) | ((lo[1] >> 16) & 0xff00u) |
-            ((lo[2] >> 8) & 0xff0000u) | (lo[3] & 0xff000000u)) ^ 0x80808080u;
-  out[4] = ((hi[0] & 0xffu) | ((hi[1] << 8) & 0xff00u) |
-            ((hi[2] << 16) & 0xff0000u) | (hi[3] << 24)) ^ 0x80808080u;
-  out[5] = ((hi[0] >> 8) & 0xffu) | (hi[1] & 0xff00u) |
-           ((hi[2] << 8) & 0xff0000u) | ((hi[3] << 16) & 0xff000000u);
+  // x01 = [lo0.b0, lo1.b0, lo0.b1, lo1.b1], y01 = [lo0.b2, lo1.b2, lo0.b3, lo1.b3]
+  const uint32_t x01 = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
+  const uint32_t y01 = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
+  const uint32_t x23 = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
+  const uint32_t y23 = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
+  const uint32_t z01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x05010400u);
+  const uint32_t z23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x05010400u);
+  out[0] = __builtin_amdgcn_perm(x23, x01, 0x05040100u) ^ 0x80808080u;
+  out[1] = __builtin_amdgcn_perm(x23, x01, 0x07060302u) ^ 0x80808080u;
+  out[2] = __builtin_amdgcn_perm(y23, y01, 0x05040100u) ^ 0x80808080u;
+  out[3] = __builtin_amdgcn_perm(y23, y01, 0x07060302u) ^ 0x80808080u;
+  out[4] = __builtin_amdgcn_perm(z23, z01, 0x05040100u) ^ 0x80808080u;
+  out[5] = __builtin_amdgcn_perm(z23, z01, 0x07060302u);
 }
 
 __device__ __forceinline__ void split_words(double v, uint32_t& lo,
@@ -103,37 +101,82 @@ __device__ __forceinline__ void split_words(double v, uint32_t& lo,
   hi = static_cast<uint32_t>(b >> 32);
 }
 
+__device__ __forceinline__ double as_double(uint32_t lo, uint32_t hi) {
+  return __longlong_as_double(static_cast<long long>(
+      static_cast<uint64_t>(lo) | (static_cast<uint64_t>(hi) << 32)));
+}
+
 // Offset correction constant: the -128 offsets of data digits 0..4 over
 // every digit pair, 128 (1 + 2^8 + 2^16 + 2^24 + 2^32) per unit of the
 // operator integers' sum.
 constexpr double kOffsetUnit = 128.0 * 4311810305.0;
-// Invalid gather offset: past the end of every C buffer the kernel reads
-// (lena * lenb * 16 < kNoRow is a launch requirement) and still a valid
-// int32 after adding a column offset (16 * lenb < 2^20).
-constexpr int32_t kNoRow = 0x7ff00000;
+// Invalid alpha gather offset, in 16-byte units: (kNoRow16 + column) * 16
+// lies past the end of C (lena * lenb * 16 <= 0x7ff00000 is a launch
+// requirement) and below 2^31 (lenb < 2^16).
+constexpr uint32_t kNoRow16 = 0x7ff0000u;
+// Beta gather table entry without an excitation (global-memory row): a
+// byte offset past the end of every row of C.
+constexpr uint32_t kNoCol = 0x7ffffff0u;
 
-// Per-q entry of the alpha gather table (LDS, per alpha string): byte
-// offset of the source row of C (kNoRow: none) and the sign of the entry
-// times the data scale 2^ed.
-struct GatherEntry {
-  int32_t off;
-  int32_t pad;
-  double sa;
-};
+// Beta gather table of the Ozaki row kernel, column-major and padded to
+// kOzKPad entries per column: tab[jb][q] = byte offset of the source column
+// of beta excitation q of column jb within its row (16 (|ptb| - 1)), with
+// the sign of the excitation in bit 31; `none` where there is none
+// (ptb == 0, or q >= npair).
+__global__ void __launch_bounds__(256)
+k_beta_table_oz(int npair, int64_t lenb, const int32_t* __restrict__ ptb,
+                uint32_t none, uint32_t* __restrict__ tab,
+                const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t total = lenb * kOzKPad;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+       i < total; i += static_cast<int64_t>(gridDim.x) * 256) {
+    const int64_t jb = i / kOzKPad;
+    const int q = static_cast<int>(i % kOzKPad);
+    uint32_t w = none;
+    if (q < npair) {
+      const int32_t b = ptb[static_cast<int64_t>(q) * lenb + jb];
+      if (b)
+        w = (b < 0 ? 0x80000000u : 0u) |
+            (static_cast<uint32_t>(iabs(b) - 1) << 4);
+    }
+    tab[i] = w;
+  }
+}
+
+// Alpha strings in an XCD-aware order: workgroup i runs on XCD i % 8, so
+// each XCD takes a contiguous range of strings, whose gather sources
+// overlap in its L2 (a bijection of [0, n)).
+__device__ __forceinline__ int64_t xcd_order(int64_t i, int64_t n) {
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = i % 8, j = i / 8;
+  return x * per + (x < rem ? x : rem) + j;
+}
+
+constexpr int kRowThreads = 512;           // 8 waves, 16 columns each
+constexpr int kRowCols = 128;              // columns per chunk
 
 // Alpha rows of the split apply (see k_split_cols_f64 in fused.hip for the
-// operands): per alpha string, the live pair rows' A' digits sit in LDS;
-// each wave owns 16 columns of a 64-column chunk, gathers their D' values
-// (lane l: column l%16, k = 64 s + 16 (l/16) + j), turns them into six
-// digit operands in registers and runs the 20 digit products (weight
-// classes m = i - t = 0..5) of each of the four 16-row tiles, the real and
-// imaginary parts one after the other.  i8 MFMA layouts: lane l holds
+// operands), one alpha string per block at a time.  Set-up: the live pair
+// rows' A' digits go to LDS and, with ROW_LDS, row ia of C too (plus one
+// zero entry at index lenb, the target of absent beta excitations); then
+// each wave owns 16 columns of a 128-column chunk, gathers their D' values
+// (lane l: column l%16, k = 64 s + 16 (l/16) + j) four k values at a time
+// with the next four's loads in flight - the alpha part from global memory
+// (absent sources read out of range: no memory access), the beta part from
+// the LDS row (random 16-byte reads, which cost about a seventh of a
+// random global gather) or, without ROW_LDS, from global memory - forms
+// the six digit operands in registers and runs the 20 digit products
+// (weight classes m = i - t = 0..5) of each of the four 16-row tiles, the
+// real and imaginary parts on the same A' operands.  The next chunk's beta
+// table words load during the products.  i8 MFMA layouts: lane l holds
 // A[row l%16][k 16 (l/16) + j] and B[k 16 (l/16) + j][col l%16] (byte j of
 // its four words); output item e of lane l is [row 4 (l/16) + e][col l%16].
-__global__ void __launch_bounds__(kOzThreads, 2)
-k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
+template <bool ROW_LDS>
+__global__ void __launch_bounds__(kRowThreads)
+k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
                 int64_t lenb, const double* __restrict__ amat,
-                const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+                const int32_t* __restrict__ pta, const uint32_t* __restrict__ btab,
                 const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
                 int nslot, const double* __restrict__ coeff,
                 const double* __restrict__ rowmax, double* __restrict__ slots,
@@ -146,10 +189,10 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
   const int K = npair;
   const int Kg = K + 1;                     // gathered rows: pairs + C
   uint8_t* adig = reinterpret_cast<uint8_t*>(smem);  // [5][kLiveRows][kOzARow]
-  GatherEntry* gtab = reinterpret_cast<GatherEntry*>(
-      adig + kOzADig * kLiveRows * kOzARow);          // [kOzKPad]
-  double* drow = reinterpret_cast<double*>(gtab + kOzKPad);  // [kOzKPad]
-  double* rscale = drow + kOzKPad;          // [kLiveRows] 2^-(ea + ed)
+  double* gsa = reinterpret_cast<double*>(adig + kOzADig * kLiveRows * kOzARow);
+  uint32_t* goff = reinterpret_cast<uint32_t*>(gsa + kOzKPad);  // [kOzKPad]
+  double* drow = reinterpret_cast<double*>(goff + kOzKPad);     // [kOzKPad]
+  double* rscale = drow + kOzKPad;          // [kLiveRows] 2^(32 - ea - ed)
   double* corr = rscale + kLiveRows;        // [kLiveRows] offset correction
   double* red = corr + kLiveRows;           // [8] block scratch
   RowTables rt;
@@ -158,17 +201,41 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
   rt.asgn = rt.aoff + kRowTabs;
   rt.ssgn = rt.asgn + kRowTabs;
   int32_t* lpair = rt.ssgn + kRowTabs;      // [kLiveRows] pair of live row
+  uint8_t* crow = reinterpret_cast<uint8_t*>(lpair + kLiveRows);  // [lenb + 1] x 16 B
 
-  const int nchunks = static_cast<int>((lenb + kOzCols - 1) / kOzCols);
+  const int nchunks = static_cast<int>((lenb + kRowCols - 1) / kRowCols);
   const __amdgpu_buffer_rsrc_t rc = buffer_of(coeff, lena * lenb * 2 * sizeof(double));
-  const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
+  const __amdgpu_buffer_rsrc_t rtb = buffer_of(btab, lenb * kOzKPad * 4);
   const int kq = lane >> 4;                 // k group of the lane
   const int cl = lane & 15;                 // column within the wave's 16
-  const int lenb4 = static_cast<int>(lenb) * 4;
 
-  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+  if (ROW_LDS && tid == 0)
+    *reinterpret_cast<u32x4*>(crow + 16 * lenb) = u32x4{0u, 0u, 0u, 0u};
+
+  // beta table words of the lane's 32 k values in a chunk:
+  // tw[s][v][u] for k = 64 s + 16 kq + 4 v + u
+  u32x4 tw[kOzKSteps][4];
+  auto load_table = [&](int chunk) {
+    const int64_t jb = static_cast<int64_t>(chunk) * kRowCols + wave * 16 + cl;
+    const int base = static_cast<int>(jb < lenb ? jb : 0) * (kOzKPad * 4) + 64 * kq;
+#pragma unroll
+    for (int s = 0; s < kOzKSteps; ++s)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        tw[s][v] = __builtin_amdgcn_raw_buffer_load_b128(rtb, base + 256 * s + 16 * v, 0, 0);
+  };
+  load_table(0);
+
+  for (int64_t it = blockIdx.x; it < lena; it += gridDim.x) {
+    const int64_t ia = xcd_order(it, lena);
     load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
-                            kOzThreads, K);
+                            kRowThreads, K);
+    if (ROW_LDS) {
+      const double* src = coeff + ia * lenb * 2;
+      for (int64_t j = tid; j < lenb; j += kRowThreads)
+        *reinterpret_cast<double2*>(crow + 16 * j) =
+            *reinterpret_cast<const double2*>(src + 2 * j);
+    }
     __syncthreads();
     if (wave == 0) {
       // compact the live rows: pairs exciting ia to another string, then
@@ -194,8 +261,8 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
       if (lane == 0) red[0] = m + 2.0 * rowmax[ia];
-    }
-    for (int k = tid; k < kOzKPad; k += kOzThreads) {
+    } else if (tid >= 256 && tid < 256 + kOzKPad) {
+      const int k = tid - 256;
       double v = 0.0;
       if (k < Kg) {
         for (int i = 0; i < norb; ++i) {
@@ -218,15 +285,12 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
       }
     }
     const double dsc = ldexp(1.0, ed);
-    if (tid < kOzKPad) {
-      const int32_t off = tid < Kg ? rt.aoff[tid] : -1;
-      GatherEntry g;
-      g.off = off >= 0 ? off * 8 : kNoRow;
-      g.pad = 0;
-      g.sa = off >= 0 ? static_cast<double>(rt.asgn[tid]) * dsc : 0.0;
-      gtab[tid] = g;
-    }
-    {
+    if (tid >= 256 && tid < 256 + kOzKPad) {
+      const int q = tid - 256;
+      const int32_t off = q < Kg ? rt.aoff[q] : -1;
+      goff[q] = off >= 0 ? static_cast<uint32_t>(off) >> 1 : kNoRow16;
+      gsa[q] = off >= 0 ? static_cast<double>(rt.asgn[q]) * dsc : 0.0;
+    } else if (tid < 256) {
       // A' digits: thread -> live row tid/4, k in [32 (tid%4), +32)
       const int m = tid >> 2;
       const int k0 = (tid & 3) * 32;
@@ -282,122 +346,126 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
         dst[1] = make_uint4(w[t][4], w[t][5], w[t][6], w[t][7]);
       }
       if ((tid & 3) == 0) {
-        corr[m] = kOffsetUnit * zsum;
-        rscale[m] = ldexp(1.0, -(ea + ed));
+        // Y = 2^(32 - ea - ed) t + 2^-(ea + ed) corr (powers of two: exact)
+        const double sc = ldexp(1.0, -(ea + ed));
+        corr[m] = kOffsetUnit * zsum * sc;
+        rscale[m] = 4294967296.0 * sc;
       }
     }
     __syncthreads();
-    const int crow_b = static_cast<int>(ia * lenb * 2 * sizeof(double));
-    double* racc = rowacc + ia * lenb * 2;
+    // row ia of C in global memory (beta gathers without ROW_LDS; offsets
+    // past the row read 0)
+    const __amdgpu_buffer_rsrc_t rrow = buffer_of(coeff + ia * lenb * 2, lenb * 16);
     const double dsc2 = 2.0 * dsc;
 
     for (int chunk = 0; chunk < nchunks; ++chunk) {
-      const int64_t jb = static_cast<int64_t>(chunk) * kOzCols + wave * 16 + cl;
+      const int64_t jb = static_cast<int64_t>(chunk) * kRowCols + wave * 16 + cl;
       const bool colok = jb < lenb;
       // padding columns gather column 0 (valid memory) and store nothing
-      const int ib = static_cast<int>(colok ? jb : 0);
-      int4v dre[kOzDDig][kOzKSteps], dim[kOzDDig][kOzKSteps];
+      const uint32_t ib = static_cast<uint32_t>(colok ? jb : 0);
+      int4v dd[2][kOzDDig][kOzKSteps];      // [re, im][digit][k step]
+      // batch b: k = 64 (b / 4) + 16 kq + 4 (b % 4) + u, u = 0..3
+      u32x4 la[2][4], lb[2][4];
+      auto issue = [&](int b) {
+        const int s = b >> 2, v = b & 3;
+        const int q0 = 64 * s + 16 * kq + 4 * v;
 #pragma unroll
-      for (int s = 0; s < kOzKSteps; ++s) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          // 8 of the lane's 16 k values of step s
-          const int q0 = 64 * s + 16 * kq + 8 * h;
-          int32_t bt[8];
-          double2 ca[8], cb[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int q = q0 + j;
-            bt[j] = bload_i32(rpb, q < K ? q * lenb4 + 4 * ib : -1);
-          }
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const GatherEntry g = gtab[q0 + j];
-            ca[j] = bload_d2(rc, g.off + 16 * ib);
-            // bt == 0 reads a neighbouring value that its zero weight drops
-            const int32_t ab = bt[j] < 0 ? -bt[j] : bt[j];
-            cb[j] = bload_d2(rc, crow_b + 16 * ab - 16);
-          }
-#pragma unroll
-          for (int g = 0; g < 2; ++g) {
-            uint32_t lre[4], hre[4], lim[4], him[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-              const int j = 4 * g + u;
-              const double sa = gtab[q0 + j].sa;
-              const int32_t sgn = bt[j] > 0 ? 1 : (bt[j] < 0 ? -1 : 0);
-              const double sb = static_cast<double>(sgn) * dsc2;
-              const double vre = fma(sa, ca[j].x, fma(sb, cb[j].x, kMagic));
-              const double vim = fma(sa, ca[j].y, fma(sb, cb[j].y, kMagic));
-              split_words(vre, lre[u], hre[u]);
-              split_words(vim, lim[u], him[u]);
-            }
-            uint32_t ore[kOzDDig], oim[kOzDDig];
-            digits_of_four(lre, hre, ore);
-            digits_of_four(lim, him, oim);
-#pragma unroll
-            for (int i = 0; i < kOzDDig; ++i) {
-              // the digit words exist here, before the next batch's loads
-              // (otherwise the conversions sink to the MFMAs and every
-              // batch's loaded values stay live at once)
-              asm volatile("" : "+v"(ore[i]), "+v"(oim[i]));
-              dre[i][s][2 * h + g] = static_cast<int>(ore[i]);
-              dim[i][s][2 * h + g] = static_cast<int>(oim[i]);
-            }
-          }
-          asm volatile("" ::: "memory");
+        for (int u = 0; u < 4; ++u) {
+          la[b & 1][u] = __builtin_amdgcn_raw_buffer_load_b128(
+              rc, static_cast<int>((goff[q0 + u] + ib) << 4), 0, 0);
+          const uint32_t o = tw[s][v][u] & 0x7fffffffu;
+          if (ROW_LDS)
+            lb[b & 1][u] = *reinterpret_cast<const u32x4*>(crow + o);
+          else
+            lb[b & 1][u] = __builtin_amdgcn_raw_buffer_load_b128(rrow, static_cast<int>(o), 0, 0);
         }
+      };
+      issue(0);
+#pragma unroll
+      for (int b = 0; b < 4 * kOzKSteps; ++b) {
+        if (b + 1 < 4 * kOzKSteps) issue(b + 1);
+        const int s = b >> 2, v = b & 3;
+        const int q0 = 64 * s + 16 * kq + 4 * v;
+        uint32_t lre[4], hre[4], lim[4], him[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const u32x4 A = la[b & 1][u];
+          const u32x4 B = lb[b & 1][u];
+          const uint32_t sg = tw[s][v][u] & 0x80000000u;
+          const double sa = gsa[q0 + u];
+          const double vre = fma(sa, as_double(A[0], A[1]),
+                                 fma(dsc2, as_double(B[0], B[1] ^ sg), kMagic));
+          const double vim = fma(sa, as_double(A[2], A[3]),
+                                 fma(dsc2, as_double(B[2], B[3] ^ sg), kMagic));
+          split_words(vre, lre[u], hre[u]);
+          split_words(vim, lim[u], him[u]);
+        }
+        uint32_t ore[kOzDDig], oim[kOzDDig];
+        digits_of_four(lre, hre, ore);
+        digits_of_four(lim, him, oim);
+#pragma unroll
+        for (int i = 0; i < kOzDDig; ++i) {
+          // the digit words exist here, before the next batch's loads
+          // (otherwise the conversions sink to the MFMAs and every
+          // batch's loaded values stay live at once)
+          asm volatile("" : "+v"(ore[i]), "+v"(oim[i]));
+          dd[0][i][s][v] = static_cast<int>(ore[i]);
+          dd[1][i][s][v] = static_cast<int>(oim[i]);
+        }
+        asm volatile("" ::: "memory");
       }
+      // the next chunk's table (chunk 0 again for the next alpha string)
+      load_table(chunk + 1 < nchunks ? chunk + 1 : 0);
 #pragma unroll 1
       for (int r = 0; r < kLiveRows / 16; ++r) {
-        double yre[4];
+        int4v acc[2][6];
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          int4v acc[6];
+        for (int c = 0; c < 2; ++c)
 #pragma unroll
-          for (int m = 0; m < 6; ++m) acc[m] = int4v{0, 0, 0, 0};
+          for (int m = 0; m < 6; ++m) acc[c][m] = int4v{0, 0, 0, 0};
 #pragma unroll
-          for (int s = 0; s < kOzKSteps; ++s) {
-            int4v a[kOzADig];
+        for (int s = 0; s < kOzKSteps; ++s) {
+          int4v a[kOzADig];
 #pragma unroll
-            for (int t = 0; t < kOzADig; ++t)
-              a[t] = *reinterpret_cast<const int4v*>(
-                  adig + (t * kLiveRows + 16 * r + cl) * kOzARow + 64 * s + 16 * kq);
-            // pairs (data digit i, operator digit t) with m = i - t >= 0
+          for (int t = 0; t < kOzADig; ++t)
+            a[t] = *reinterpret_cast<const int4v*>(
+                adig + (t * kLiveRows + 16 * r + cl) * kOzARow + 64 * s + 16 * kq);
+          // pairs (data digit i, operator digit t) with m = i - t >= 0
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int i = 0; i < kOzDDig; ++i)
 #pragma unroll
               for (int t = 0; t < kOzADig; ++t)
                 if (t <= i)
-                  acc[i - t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                      a[t], c ? dim[i][s] : dre[i][s], acc[i - t], 0, 0, 0);
-          }
-          if (colok) {
+                  acc[c][i - t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                      a[t], dd[c][i][s], acc[c][i - t], 0, 0, 0);
+        }
+        if (colok) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-              const int row = 16 * r + 4 * kq + e;
-              double t = static_cast<double>(acc[5][e]);
+          for (int e = 0; e < 4; ++e) {
+            const int row = 16 * r + 4 * kq + e;
+            double y[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              double t = static_cast<double>(acc[c][5][e]);
 #pragma unroll
               for (int m = 4; m >= 0; --m)
-                t = fma(t, 256.0, static_cast<double>(acc[m][e]));
-              const double y = fma(t, 4294967296.0, corr[row]) * rscale[row];
-              if (c == 0) {
-                yre[e] = y;
-              } else {
-                const int po = lpair[row];
-                if (po == npair) {
-                  double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
-                  double2 o = *d;
-                  o.x += yre[e];
-                  o.y += y;
-                  *d = o;
-                } else if (po >= 0) {
-                  const int64_t sb = rt.sbase[po];
-                  const double sg = static_cast<double>(rt.ssgn[po]);
-                  *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
-                      make_double2(sg * yre[e], sg * y);
-                }
-              }
+                t = fma(t, 256.0, static_cast<double>(acc[c][m][e]));
+              y[c] = fma(t, rscale[row], corr[row]);
+            }
+            const int po = lpair[row];
+            if (po == npair) {
+              double2* d = reinterpret_cast<double2*>(rowacc + ia * lenb * 2 + 2 * jb);
+              double2 o = *d;
+              o.x += y[0];
+              o.y += y[1];
+              *d = o;
+            } else if (po >= 0) {
+              const int64_t sb = rt.sbase[po];
+              const double sg = static_cast<double>(rt.ssgn[po]);
+              *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
+                  make_double2(sg * y[0], sg * y[1]);
             }
           }
         }
@@ -407,13 +475,16 @@ k_split_cols_oz(int norb, int compressed, int npair, int64_t lena,
   }
 }
 
-size_t split_cols_oz_lds_bytes() {
+size_t split_rows_oz_lds_bytes(bool row_lds, int64_t lenb) {
   return static_cast<size_t>(kOzADig) * kLiveRows * kOzARow +
-         kOzKPad * sizeof(GatherEntry) +
+         kOzKPad * (sizeof(double) + sizeof(uint32_t)) +
          (kOzKPad + 2 * kLiveRows + 8) * sizeof(double) +
          kRowTabs * (sizeof(int64_t) + 3 * sizeof(int32_t)) +
-         kLiveRows * sizeof(int32_t);
+         kLiveRows * sizeof(int32_t) +
+         (row_lds ? static_cast<size_t>(lenb + 1) * 16 : 0);
 }
+
+constexpr size_t kLdsLimit = 160 * 1024;
 
 }  // namespace
 
@@ -421,29 +492,48 @@ size_t split_cols_oz_lds_bytes() {
 // split kernel: npair + 1 <= 128, norb^2/4 + 1 <= 64 live rows, 32-bit byte
 // offsets); rowmax: the row maxima of |C| (launch_row_absmax first).
 bool split_oz_supported(int norb, int npair, int64_t lena, int64_t lenb) {
-  const int64_t lim = int64_t(1) << 31;
   return npair + 1 <= kOzKPad && norb * norb / 4 + 1 <= kLiveRows &&
-         static_cast<int64_t>(npair) * lenb * 4 < lim &&
-         lena * lenb * 2 * 8 < kNoRow && lenb < (1 << 16);
+         lenb < (1 << 16) &&
+         lena * lenb * 16 <= static_cast<int64_t>(kNoRow16) * 16;
 }
+
+// Bytes of the beta gather table (btab) of launch_split_oz.
+int64_t split_oz_table_bytes(int64_t lenb) { return lenb * kOzKPad * 4; }
 
 int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
                     const int32_t* fwa, const int32_t* sla, int nslot,
-                    const double* coeff, const double* rowmax, double* slots,
-                    double* rowacc, hipStream_t st, const int32_t* done) {
+                    const double* coeff, const double* rowmax, void* btab,
+                    double* slots, double* rowacc, hipStream_t st,
+                    const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   if (!split_oz_supported(norb, npair, lena, lenb)) {
-    set_error("the Ozaki split apply needs npair + 1 <= %d and norb^2/4 + 1 <= %d",
-              kOzKPad, kLiveRows);
+    set_error("the Ozaki split apply needs npair + 1 <= %d, norb^2/4 + 1 <= %d, "
+              "lenb < 65536 and 16 lena lenb <= %lld",
+              kOzKPad, kLiveRows, static_cast<long long>(kNoRow16) * 16);
     return FQEHIP_ERR_INVALID;
   }
+  uint32_t* tab = static_cast<uint32_t*>(btab);
+  const bool row_lds = split_rows_oz_lds_bytes(true, lenb) <= kLdsLimit;
+  const int64_t entries = lenb * kOzKPad;
+  const unsigned tgrid = static_cast<unsigned>(std::min<int64_t>((entries + 255) / 256, 4096));
+  // absent beta excitations: the zero entry after the LDS row, or past the
+  // end of the row in global memory
+  const uint32_t none = row_lds ? static_cast<uint32_t>(16 * lenb) : kNoCol;
+  k_beta_table_oz<<<tgrid, 256, 0, st>>>(npair, lenb, ptb, none, tab, done);
+  FQEHIP_CHECK_LAUNCH("k_beta_table_oz");
   const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
   const unsigned grid = static_cast<unsigned>(lena < 65536 ? lena : 65536);
-  k_split_cols_oz<<<grid, kOzThreads, split_cols_oz_lds_bytes(), st>>>(
-      norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot, coeff,
-      rowmax, slots, rowacc, done);
-  FQEHIP_CHECK_LAUNCH("k_split_cols_oz");
+  const size_t lds = split_rows_oz_lds_bytes(row_lds, lenb);
+  if (row_lds)
+    k_split_rows_oz<true><<<grid, kRowThreads, lds, st>>>(
+        norb, compressed, npair, lena, lenb, a, pta, tab, fwa, sla, nslot,
+        coeff, rowmax, slots, rowacc, done);
+  else
+    k_split_rows_oz<false><<<grid, kRowThreads, lds, st>>>(
+        norb, compressed, npair, lena, lenb, a, pta, tab, fwa, sla, nslot,
+        coeff, rowmax, slots, rowacc, done);
+  FQEHIP_CHECK_LAUNCH("k_split_rows_oz");
   return FQEHIP_OK;
 }
 
