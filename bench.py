@@ -69,6 +69,8 @@ def parse():
                          "ozaki (the headline), bf16x3")
     ap.add_argument("--no-cfg1", action="store_true",
                     help="skip the norb=12 chained-apply workload")
+    ap.add_argument("--no-evolve", action="store_true",
+                    help="skip the timed time_evolve lines")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="first copy this many bytes device-to-device (the "
                          "calibration kernel of HBM traffic counters)")
@@ -795,6 +797,98 @@ def cfg1_line(with_cpu, chain=100, cpu_chain=10):
     return line
 
 
+def _timed_evolve(wfn, ham, t, repeats=3):
+    """Median wall time of wfn.time_evolve(t, ham) over ``repeats`` calls
+    after one untimed call (tables, operator, graph recording), and the
+    last result."""
+    import statistics
+    import torch
+    ev = wfn.time_evolve(t, ham)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        ev = wfn.time_evolve(t, ham)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    return statistics.median(times), ev
+
+
+def time_evolve_lines(with_cpu):
+    """Timed time_evolve (Taylor, accuracy 1e-15, expansion 30) on the
+    Wavefunction API with C resident in HBM, per engine, eager and with the
+    series recorded in one HIP graph:
+    - norb=8 4a4b (4,900 determinants), seed 0, 8 factors: one apply plus
+      time_evolve(0.1), against the reference's compiled C on the host;
+    - norb=14 7a7b, seed 2, 14 factors: time_evolve(0.05) (the reference's
+      C would take about a minute per series here, so no CPU leg)."""
+    import numpy
+    import torch
+    import fqe_amd as fqe
+    from fqe_amd import settings, synthetic
+    saved = settings.contraction, settings.taylor_graph
+    out = {}
+    try:
+        for name, (norb, nel, seed, t) in {"norb8_4a4b": (8, 4, 0, 0.1),
+                                          "norb14_7a7b": (14, 7, 2, 0.05)
+                                          }.items():
+            c, h1, h2 = synthetic.workload(norb, nel, nel, seed, norb)
+            key = (2 * nel, 0)
+            ham = fqe.get_restricted_hamiltonian((h1, h2))
+            wfn = fqe.Wavefunction([[2 * nel, 0, norb]])
+            wfn.set_wfn(strategy="from_data", raw_data={key: c})
+            line = {"workload": (f"norb={norb} {nel}a{nel}b ({c.size} "
+                                 f"determinants), seed {seed}, {norb} "
+                                 f"factors: time_evolve({t}), Taylor, "
+                                 "accuracy 1e-15, expansion 30"),
+                    "timing": "median of 3 calls after one untimed call"}
+            res = {}
+            for eng, graph in (("fp64", False), ("fp64", True),
+                               ("ozaki", False), ("ozaki", True)):
+                settings.contraction, settings.taylor_graph = eng, graph
+                try:
+                    sec, ev = _timed_evolve(wfn, ham, t)
+                except RuntimeError as err:
+                    line[f"{eng}{'_graph' if graph else ''}"] = str(err)
+                    continue
+                tag = f"{eng}{'_graph' if graph else ''}"
+                line[f"gpu_{tag}_s"] = round(sec, 5)
+                line["taylor_terms"] = int(ev._taylor_terms)
+                res[tag] = ev.get_coeff(key)
+            if "fp64" in res and "ozaki" in res:
+                line["rel_err_ozaki_vs_fp64"] = float(
+                    numpy.linalg.norm(res["ozaki"] - res["fp64"]) /
+                    numpy.linalg.norm(res["fp64"]))
+            settings.contraction, settings.taylor_graph = "fp64", False
+            if norb == 8:
+                t0 = time.perf_counter()
+                wfn.apply(ham).get_coeff(key)
+                line["gpu_fp64_apply_host_arrays_s"] = round(
+                    time.perf_counter() - t0, 5)
+            if with_cpu and norb == 8:
+                from oracle import ref_lm
+                if ref_lm.available():
+                    g = ref_lm.RefGraph(nel, nel, norb)
+                    t0 = time.perf_counter()
+                    ref_lm.apply_12_lm(c, h1, h2, g)
+                    ref_ev, nterms = _ref_taylor(c, h1, h2, g, t)
+                    line["cpu_reference_apply_plus_evolve_s"] = round(
+                        time.perf_counter() - t0, 4)
+                    line["reference_taylor_terms"] = int(nterms)
+                    line["cpu_sample"] = (f"one apply + the whole series, "
+                                          f"{HOST_CORES} host cores")
+                    for tag, v in res.items():
+                        line[f"rel_err_{tag}_vs_reference"] = float(
+                            numpy.linalg.norm(v - ref_ev) /
+                            numpy.linalg.norm(ref_ev))
+            out[name] = line
+            del wfn
+            torch.cuda.empty_cache()
+    finally:
+        settings.contraction, settings.taylor_graph = saved
+    return out
+
+
 def main():
     args = parse()
     if args.batched and args.gpus == 1:
@@ -875,6 +969,9 @@ def main():
     if not args.no_cfg1 and (norb, na, nb) == (14, 7, 7):
         cfg1 = cfg1_line(not args.no_cpu_baseline)
     host_rate = host_boundary_rate(coeff, h1, h2, norb, na, nb)
+    evolve = None
+    if not args.no_evolve and (norb, na, nb) == (14, 7, 7):
+        evolve = time_evolve_lines(not args.no_cpu_baseline)
 
     oz, fp, bf = results["ozaki"], results["fp64"], results["bf16x3"]
     value = args.steps / oz["wall"]
@@ -941,6 +1038,8 @@ def main():
                                                          ref_out)
     if cfg1 is not None:
         line["cfg1"] = cfg1
+    if evolve is not None:
+        line["time_evolve"] = evolve
     print(json.dumps(line))
 
 

@@ -378,9 +378,7 @@ __device__ __forceinline__ void fused_row(
                 const int64_t sb = rt.sbase[p];
                 if (sb >= 0) {
                   const float sg = static_cast<float>(rt.ssgn[p]);
-                  float* dst = slots + sb + 2 * jb;
-                  dst[0] = sg * are[r][i];
-                  dst[1] = sg * aim[r][i];
+                  stream_store2(slots + sb + 2 * jb, sg * are[r][i], sg * aim[r][i]);
                 }
               }
             } else if (p <= npair && jb < lenb) {
@@ -631,8 +629,8 @@ k_split_live(int norb, int compressed, int npair, int64_t lena, int64_t lenb,
             for (int i = 0; i < 4; ++i) {
               const int64_t sb = obase[r][i];
               if (sb >= 0) {
-                *reinterpret_cast<float2*>(slots + sb + 2 * jb) =
-                    make_float2(osgn[r][i] * are[r][i], osgn[r][i] * aim[r][i]);
+                stream_store2(slots + sb + 2 * jb, osgn[r][i] * are[r][i],
+                              osgn[r][i] * aim[r][i]);
               } else if (sb == -3) {
                 float2* d = reinterpret_cast<float2*>(racc + 2 * jb);
                 float2 o = *d;
@@ -975,8 +973,8 @@ k_split_live_f64(int norb, int compressed, int npair, int64_t lena,
           const int64_t sb = obase[i];
           if (sb >= 0) {
             const double sg = (sb & 1) ? -1.0 : 1.0;
-            *reinterpret_cast<double2*>(slots + (sb & ~int64_t(1)) + 2 * jb) =
-                make_double2(sg * acc_re[i], sg * acc_im[i]);
+            stream_store2(slots + (sb & ~int64_t(1)) + 2 * jb, sg * acc_re[i],
+                          sg * acc_im[i]);
           } else if (sb == -3) {
             double2* d = reinterpret_cast<double2*>(racc + 2 * jb);
             double2 o = *d;
@@ -1163,8 +1161,7 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
             } else if (po >= 0) {
               const int64_t sb = rt.sbase[po];
               const double sg = static_cast<double>(rt.ssgn[po]);
-              *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
-                  make_double2(sg * accr[r][i], sg * acci[r][i]);
+              stream_store2(slots + sb + 2 * jb, sg * accr[r][i], sg * acci[r][i]);
             }
           }
       }
@@ -1192,12 +1189,15 @@ __device__ __forceinline__ double slot_element(
   // keep one, which measured faster); the FP64 sum keeps slot order
   constexpr int kU = sizeof(T) == 4 ? 8 : 1;
   int k = 0;
+  typedef T t2v __attribute__((ext_vector_type(2)));
   for (; k + kU <= nslot; k += kU) {
     T vr[kU], vi[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      vr[u] = s[2 * lenb * u];
-      vi[u] = s[2 * lenb * u + 1];
+      const t2v v = __builtin_nontemporal_load(
+          reinterpret_cast<const t2v*>(s + 2 * lenb * u));
+      vr[u] = v[0];
+      vi[u] = v[1];
     }
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -1207,8 +1207,9 @@ __device__ __forceinline__ double slot_element(
     s += 2 * lenb * kU;
   }
   for (; k < nslot; ++k) {
-    re += static_cast<double>(s[0]);
-    im += static_cast<double>(s[1]);
+    const t2v v = __builtin_nontemporal_load(reinterpret_cast<const t2v*>(s));
+    re += static_cast<double>(v[0]);
+    im += static_cast<double>(v[1]);
     s += 2 * lenb;
   }
   double vr = alpha_re * re - alpha_im * im;

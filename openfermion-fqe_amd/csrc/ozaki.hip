@@ -464,8 +464,7 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
             } else if (po >= 0) {
               const int64_t sb = rt.sbase[po];
               const double sg = static_cast<double>(rt.ssgn[po]);
-              *reinterpret_cast<double2*>(slots + sb + 2 * jb) =
-                  make_double2(sg * y[0], sg * y[1]);
+              stream_store2(slots + sb + 2 * jb, sg * y[0], sg * y[1]);
             }
           }
         }

@@ -41,6 +41,18 @@ __device__ __forceinline__ double2 bload_d2(__amdgpu_buffer_rsrc_t r, int off) {
                       __longlong_as_double(static_cast<long long>(hi)));
 }
 
+// Streaming (non-temporal) stores and loads of one complex value: the slot
+// rows are written once by the row kernels and read once by the reduction,
+// and should not evict the gathered C rows from L2 and the last-level cache.
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stream_store2(double* p, double x, double y) {
+  __builtin_nontemporal_store(d2v{x, y}, reinterpret_cast<d2v*>(p));
+}
+__device__ __forceinline__ void stream_store2(float* p, float x, float y) {
+  __builtin_nontemporal_store(f2v{x, y}, reinterpret_cast<f2v*>(p));
+}
+
 constexpr int kRowTabs = 128;              // per-row LDS tables (>= npair + 1)
 
 struct RowTables {
