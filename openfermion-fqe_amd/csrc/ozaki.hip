@@ -555,9 +555,12 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
 namespace fqehip {
 namespace {
 
-constexpr int kBgT = 64;                    // tile rows (M) and columns (N)
+constexpr int kBgTM = 128;                  // tile rows (M)
+constexpr int kBgTN = 64;                   // tile columns (N)
 constexpr int kBgK = 64;                    // K step
 constexpr int kBgS = kBgK + 16;             // bytes per LDS row (padding)
+constexpr int kBgC = 6 * kBgTM * 4;         // 16-byte C staging loads per K step
+constexpr int kBgH = 5 * kBgTN * 4;         // 16-byte Hbb staging loads per K step
 constexpr int kBgXcds = 8;                  // XCDs of an MI355X
 constexpr int kBgGroup = 8;                 // row tiles per group of the order
 
@@ -676,11 +679,15 @@ k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
   hcorr[c] = corr;
 }
 
-// GEMM tile: 64 M rows (32 alpha strings, real and imaginary rows) x 64
-// columns per workgroup of 4 waves (2 x 2, 32 x 32 per wave: 2 x 2 MFMA
-// tiles of 16 x 16, six accumulators each), K in steps of 64 staged
-// through LDS with the next step's global loads in flight.
-__global__ void __launch_bounds__(256, 2)
+// GEMM tile: 128 M rows (64 alpha strings, real and imaginary rows) x 64
+// columns per workgroup of 4 waves (2 x 2, 64 x 32 per wave: 4 x 2 MFMA
+// tiles of 16 x 16, six accumulators each: 192 accumulator registers, so
+// one wave per SIMD with the accumulators in AGPRs), K in steps of 64
+// staged through LDS (87 KB, one workgroup per CU) with the next step's
+// global loads in flight.  Against 32 x 32 wave tiles at two waves per SIMD
+// this halves the LDS operand reads per MFMA (each A fragment serves two
+// column tiles, each B fragment four row tiles) and the barriers per MFMA.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
                const uint8_t* __restrict__ cdig,
                const double* __restrict__ cscale,
@@ -688,8 +695,9 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
                const double* __restrict__ hscale,
                const double* __restrict__ hcorr, double* __restrict__ out,
                const int32_t* __restrict__ done, int tm, int tn, int per_xcd) {
-  __shared__ __align__(16) uint8_t sc[6][kBgT * kBgS];
-  __shared__ __align__(16) uint8_t sh[5][kBgT * kBgS];
+  extern __shared__ __align__(16) uint8_t bg_smem[];
+  uint8_t* sc = bg_smem;                              // [6][kBgTM][kBgS]
+  uint8_t* sh = bg_smem + 6 * kBgTM * kBgS;           // [5][kBgTN][kBgS]
   if (done && *done) return;
   int im, in;
   if (!oz_tile(tm, tn, per_xcd, &im, &in)) return;
@@ -697,41 +705,48 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int64_t rows = 2 * lena;
-  const int64_t m0 = static_cast<int64_t>(im) * kBgT;
-  const int64_t n0 = static_cast<int64_t>(in) * kBgT;
-  const int wm = (wave >> 1) * 32;
+  const int64_t m0 = static_cast<int64_t>(im) * kBgTM;
+  const int64_t n0 = static_cast<int64_t>(in) * kBgTN;
+  const int wm = (wave >> 1) * 64;
   const int wn = (wave & 1) * 32;
   const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 6 * rows * ldk);
   const __amdgpu_buffer_rsrc_t rhd = buffer_of(hdig, 5 * lenb * ldk);
-  // staging: thread -> (tile row tid/4, 16-byte segment tid%4) of each plane
-  const int sr = tid >> 2;
-  const int sseg = 16 * (tid & 3);
-  const bool crow_ok = m0 + sr < rows;
-  const bool hrow_ok = n0 + sr < lenb;
-  u32x4 cv[6], hv[5];
+  // staging: load j of a thread -> (plane, tile row, 16-byte segment) of
+  // index tid + 256 j in plane-major order
+  u32x4 cv[kBgC / 256], hv[kBgH / 256];
   auto load = [&](int64_t k0) {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      const int oc = crow_ok ? static_cast<int>((i * rows + m0 + sr) * ldk + k0 + sseg) : -1;
-      cv[i] = __builtin_amdgcn_raw_buffer_load_b128(rcd, oc, 0, 0);
+    for (int j = 0; j < kBgC / 256; ++j) {
+      const int x = tid + 256 * j;
+      const int i = x / (kBgTM * 4), r = (x / 4) % kBgTM, g = x % 4;
+      const int oc = m0 + r < rows ? static_cast<int>((i * rows + m0 + r) * ldk + k0 + 16 * g) : -1;
+      cv[j] = __builtin_amdgcn_raw_buffer_load_b128(rcd, oc, 0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-      const int oh = hrow_ok ? static_cast<int>((i * lenb + n0 + sr) * ldk + k0 + sseg) : -1;
-      hv[i] = __builtin_amdgcn_raw_buffer_load_b128(rhd, oh, 0, 0);
+    for (int j = 0; j < kBgH / 256; ++j) {
+      const int x = tid + 256 * j;
+      const int i = x / (kBgTN * 4), r = (x / 4) % kBgTN, g = x % 4;
+      const int oh = n0 + r < lenb ? static_cast<int>((i * lenb + n0 + r) * ldk + k0 + 16 * g) : -1;
+      hv[j] = __builtin_amdgcn_raw_buffer_load_b128(rhd, oh, 0, 0);
     }
   };
   auto store = [&]() {
 #pragma unroll
-    for (int i = 0; i < 6; ++i)
-      *reinterpret_cast<u32x4*>(&sc[i][sr * kBgS + sseg]) = cv[i];
+    for (int j = 0; j < kBgC / 256; ++j) {
+      const int x = tid + 256 * j;
+      const int i = x / (kBgTM * 4), r = (x / 4) % kBgTM, g = x % 4;
+      *reinterpret_cast<u32x4*>(sc + (i * kBgTM + r) * kBgS + 16 * g) = cv[j];
+    }
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
-      *reinterpret_cast<u32x4*>(&sh[i][sr * kBgS + sseg]) = hv[i];
+    for (int j = 0; j < kBgH / 256; ++j) {
+      const int x = tid + 256 * j;
+      const int i = x / (kBgTN * 4), r = (x / 4) % kBgTN, g = x % 4;
+      *reinterpret_cast<u32x4*>(sh + (i * kBgTN + r) * kBgS + 16 * g) = hv[j];
+    }
   };
-  int4v acc[2][2][6];
+  int4v acc[4][2][6];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -743,22 +758,22 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   __syncthreads();
   for (int64_t ks = 0; ks < nk; ++ks) {
     if (ks + 1 < nk) load((ks + 1) * kBgK);
-    int4v a[2][6];
+    int4v a[4][6];
 #pragma unroll
-    for (int x = 0; x < 2; ++x)
+    for (int x = 0; x < 4; ++x)
 #pragma unroll
       for (int i = 0; i < 6; ++i)
         a[x][i] = *reinterpret_cast<const int4v*>(
-            &sc[i][(wm + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
+            sc + (i * kBgTM + wm + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4));
 #pragma unroll
     for (int y = 0; y < 2; ++y) {
       int4v b[5];
 #pragma unroll
       for (int i = 0; i < 5; ++i)
         b[i] = *reinterpret_cast<const int4v*>(
-            &sh[i][(wn + 16 * y + (lane & 15)) * kBgS + 16 * (lane >> 4)]);
+            sh + (i * kBgTN + wn + 16 * y + (lane & 15)) * kBgS + 16 * (lane >> 4));
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int i = 0; i < 6; ++i)
 #pragma unroll
@@ -776,7 +791,7 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   // output item e of lane l: M row 4 (l/16) + e, column l%16; rows 2 Ia
   // (real) and 2 Ia + 1 (imaginary) are items e and e + 1
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < 4; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) {
       const int64_t col = n0 + wn + 16 * y + (lane & 15);
@@ -803,6 +818,8 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
       }
     }
 }
+
+constexpr size_t kBgLds = static_cast<size_t>(6 * kBgTM + 5 * kBgTN) * kBgS;
 
 }  // namespace
 
@@ -835,10 +852,10 @@ int launch_beta_gemm_oz(int64_t lena, int64_t lenb, const void* hoz,
       lena, lenb, ldk, coeff, rowmax, reinterpret_cast<uint32_t*>(cdig), cscale,
       done);
   FQEHIP_CHECK_LAUNCH("k_c_digits");
-  const int tm = static_cast<int>((2 * lena + kBgT - 1) / kBgT);
-  const int tn = static_cast<int>((lenb + kBgT - 1) / kBgT);
+  const int tm = static_cast<int>((2 * lena + kBgTM - 1) / kBgTM);
+  const int tn = static_cast<int>((lenb + kBgTN - 1) / kBgTN);
   const int per_xcd = (tm * tn + kBgXcds - 1) / kBgXcds;
-  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 256, 0, st>>>(
+  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 256, kBgLds, st>>>(
       lena, lenb, ldk, cdig, cscale, hdig, hscale, hcorr, out, done, tm, tn,
       per_xcd);
   FQEHIP_CHECK_LAUNCH("k_beta_gemm_oz");
