@@ -15,7 +15,7 @@ from .fci_graph import FciGraph
 from .fqe_data import FqeData
 from .fqe_data_set import FqeDataSet
 from .hamiltonians import (Hamiltonian, RestrictedHamiltonian, General,
-                           GSOHamiltonian)
+                           GSOHamiltonian, SSOHamiltonian)
 from .util import vdot, alpha_beta_electrons
 from .wavefunction import Wavefunction as _Wavefunction
 from .wavefunction import build_hamiltonian
@@ -42,6 +42,11 @@ def get_general_hamiltonian(tensors, e_0: complex = 0. + 0.j) -> General:
 def get_gso_hamiltonian(tensors, e_0: complex = 0. + 0.j) -> GSOHamiltonian:
     """reference ``src/fqe/_fqe_control.py:542-556``"""
     return GSOHamiltonian(tensors, e_0=e_0)
+
+
+def get_sso_hamiltonian(tensors, e_0: complex = 0. + 0.j) -> SSOHamiltonian:
+    """reference ``src/fqe/_fqe_control.py:599-613``"""
+    return SSOHamiltonian(tensors, e_0=e_0)
 
 
 def get_restricted_hamiltonian(tensors, e_0: complex = 0. + 0.j

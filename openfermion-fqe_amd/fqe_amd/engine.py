@@ -30,13 +30,17 @@ class PreparedOperator:
     """GEMM matrix (2-body) or pair vector (1-body) on the device."""
 
     def __init__(self, norb: int, compressed: bool, complex_op: bool,
-                 a: Optional[torch.Tensor], h1: Optional[torch.Tensor]):
+                 a: Optional[torch.Tensor], h1: Optional[torch.Tensor],
+                 spin: bool = False):
         self.norb = norb
         self.compressed = compressed
         self.complex_op = complex_op
         self.a = a
         self.h1 = h1
-        self.npair = norb * (norb + 1) // 2 if compressed else norb * norb
+        # spin: pairs of alpha orbitals, then pairs of beta orbitals
+        self.spin = spin
+        self.npair = (2 * norb * norb if spin else
+                      norb * (norb + 1) // 2 if compressed else norb * norb)
         # A[p][q] == A[q][p] exactly (p, q < npair): the split apply applies
         self.pair_symmetric = False
         self._hbb = {}
@@ -137,15 +141,7 @@ def prepare_operator(tensors: Sequence[numpy.ndarray], norb: int,
     vm = v.reshape(norb * norb, norb * norb)[numpy.ix_(rows, rows)]
     h1row = h1p.reshape(-1)[rows]
     full = numpy.vstack([vm, h1row[None, :]])  # (npair+1) x npair, complex
-    if complex_op:
-        m, k = full.shape
-        a = numpy.zeros((2 * m, 2 * k))
-        a[0::2, 0::2] = full.real
-        a[0::2, 1::2] = -full.imag
-        a[1::2, 0::2] = full.imag
-        a[1::2, 1::2] = full.real
-    else:
-        a = numpy.ascontiguousarray(full.real)
+    a = _real_block_form(full, complex_op)
     dev_a = torch.from_numpy(numpy.ascontiguousarray(a, dtype=numpy.float64)
                              ).to(device)
     prepared = PreparedOperator(norb, compressed, complex_op, dev_a, None)
@@ -153,6 +149,78 @@ def prepare_operator(tensors: Sequence[numpy.ndarray], norb: int,
         sq = a[:-1, :]
         prepared.pair_symmetric = bool(numpy.array_equal(sq, sq.T))
     return prepared
+
+
+def _real_block_form(full: numpy.ndarray, complex_op: bool) -> numpy.ndarray:
+    """The GEMM matrix of a complex (npair+1) x npair operator: itself when
+    real, else the real 2 x 2 block form on planar (re, im) rows."""
+    if not complex_op:
+        return numpy.ascontiguousarray(full.real)
+    m, k = full.shape
+    a = numpy.zeros((2 * m, 2 * k))
+    a[0::2, 0::2] = full.real
+    a[0::2, 1::2] = -full.imag
+    a[1::2, 0::2] = full.imag
+    a[1::2, 1::2] = full.real
+    return a
+
+
+def prepare_spin_operator(tensors: Sequence[numpy.ndarray], norb: int,
+                          device: torch.device) -> PreparedOperator:
+    """Device form of a 1- or 2-tensor spin-orbital operator (2 norb
+    spin orbitals, alpha first) applied to one (nalpha, nbeta) sector, in
+    the reference's dvec form with separate alpha and beta dvecs
+    (``src/fqe/fqe_data.py:533-583`` one-body, ``:712-741`` two-body):
+
+        out = sum_ij h1'aa[i,j] Da[i,j] + h1'bb[i,j] Db[i,j]
+              + sum_ij E^a_ij Ya[i,j] + E^b_ij Yb[i,j],
+        Ya = V'aaaa Da + V'aabb Db,  Yb = V'bbaa Da + V'bbbb Db,
+
+    with V' = -moveaxis(h2e, 1, 2) and h1' = h1e - sum_k V'[i,k,k,j] over
+    all 2 norb spin orbitals.  Only the spin-conserving blocks enter, as in
+    the reference; the alpha-beta blocks of h1e are not used.  The pair
+    basis is the full norb^2 basis of each spin, alpha pairs first: a
+    (2 norb^2 + 1) x 2 norb^2 GEMM matrix (the last row h1'), or a pair
+    vector for a one-body operator."""
+    n = norb
+    m = 2 * n
+    npair = n * n
+    if len(tensors) == 1:
+        h1 = numpy.asarray(tensors[0], dtype=numpy.complex128)
+        if h1.shape != (m, m):
+            raise ValueError("Inconsistent number of spin-orbitals in "
+                             "operators and wavefunction.")
+        vec = numpy.concatenate([h1[:n, :n].reshape(-1),
+                                 h1[n:, n:].reshape(-1)])
+        d = torch.from_numpy(vec.view(numpy.float64).copy()).to(device)
+        return PreparedOperator(norb, False, bool(numpy.any(vec.imag)), None,
+                                d, spin=True)
+    if len(tensors) != 2:
+        raise ValueError("the GPU engine applies 1- and 2-body spin-orbital "
+                         "operators to spin-conserving sectors")
+    h1e, h2e = tensors
+    if h1e is None:
+        h1e = numpy.zeros((m, m))
+    h1e = numpy.asarray(h1e)
+    h2e = numpy.asarray(h2e)
+    assert h1e.shape == (m, m)
+    assert h2e.shape == (m, m, m, m)
+    v = -numpy.moveaxis(h2e.astype(numpy.complex128), 1, 2)
+    h1p = h1e.astype(numpy.complex128) - numpy.einsum("ikkj->ij", v)
+    vm = numpy.zeros((2 * npair, 2 * npair), dtype=numpy.complex128)
+    for s in range(2):
+        for t in range(2):
+            vm[s * npair:(s + 1) * npair, t * npair:(t + 1) * npair] = \
+                v[s * n:(s + 1) * n, s * n:(s + 1) * n,
+                  t * n:(t + 1) * n, t * n:(t + 1) * n].reshape(npair, npair)
+    h1row = numpy.concatenate([h1p[:n, :n].reshape(-1),
+                               h1p[n:, n:].reshape(-1)])
+    full = numpy.vstack([vm, h1row[None, :]])
+    complex_op = bool(numpy.any(full.imag))
+    a = _real_block_form(full, complex_op)
+    dev_a = torch.from_numpy(numpy.ascontiguousarray(a, dtype=numpy.float64)
+                             ).to(device)
+    return PreparedOperator(norb, False, complex_op, dev_a, None, spin=True)
 
 
 def precision_of(dtype: torch.dtype) -> int:
@@ -171,11 +239,15 @@ def sector_struct(graph, op: PreparedOperator,
     beta-beta block is attached (split apply): its bfloat16 hi/lo split for
     complex64 (``settings.split``), its FP64 columns for complex128
     (``settings.split_fp64``)."""
-    ta = graph.tables(True).pair_table(op.compressed)
-    tb = graph.tables(False).pair_table(op.compressed)
+    if op.spin:
+        ta = graph.tables(True).spin_pair_table(True)
+        tb = graph.tables(False).spin_pair_table(False)
+    else:
+        ta = graph.tables(True).pair_table(op.compressed)
+        tb = graph.tables(False).pair_table(op.compressed)
     keep = [ta, tb]
     fwa = fwb = sla = None
-    if settings.fused and not op.complex_op and op.h1 is None:
+    if settings.fused and not op.complex_op and op.h1 is None and not op.spin:
         fwa, sla = graph.tables(True).fused_tables(op.compressed)
         fwb, _ = graph.tables(False).fused_tables(op.compressed)
         keep += [fwa, fwb, sla]

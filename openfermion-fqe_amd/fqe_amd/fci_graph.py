@@ -58,6 +58,19 @@ class SpinTables:
             self._ptab[key] = tab
         return self._ptab[key]
 
+    def spin_pair_table(self, first: bool) -> torch.Tensor:
+        """int32 [2 norb^2][length]: the full-basis pair table of this spin
+        in the first (alpha, ``first``) or second (beta) half and zeros in
+        the other, so that pair p < norb^2 excites alpha strings only and
+        p >= norb^2 beta strings only (spin-orbital operators on one
+        sector)."""
+        key = ("spin", bool(first))
+        if key not in self._ptab:
+            pt = self.pair_table(False)
+            zero = torch.zeros_like(pt)
+            self._ptab[key] = torch.cat([pt, zero] if first else [zero, pt])
+        return self._ptab[key]
+
     def fused_tables(self, compressed: bool):
         """(forward table, slot table) of the fused apply, int32
         [npair][length] each (see fqehip_fused_tables)."""

@@ -174,13 +174,20 @@ class FqeData:
         if not spatial and first.shape[0] != 2 * self.norb():
             raise ValueError("Inconsistent number of spin-orbitals in "
                              "operators and wavefunction.")
-        if not spatial or len_arr > 2:
+        if len_arr > 2:
             raise NotImplementedError(
-                "the GPU engine applies spatial 1- and 1+2-body operators "
-                "(RestrictedHamiltonian); spin-orbital and 3/4-body "
-                "operators are outside its path")
+                "the GPU engine applies 1- and 1+2-body operators; 3- and "
+                "4-body operators are outside its path")
         if op is None:
-            op = engine.prepare_operator(array, self.norb(), _lib.device())
+            if spatial:
+                op = engine.prepare_operator(array, self.norb(),
+                                             _lib.device())
+            else:
+                # spin-orbital tensors on this sector (reference
+                # _apply_array_spin1 / _apply_array_spin12,
+                # src/fqe/fqe_data.py:533-583, :610-635)
+                op = engine.prepare_spin_operator(array, self.norb(),
+                                                  _lib.device())
         cin = self.device_coeff()
         out = torch.empty_like(cin)
         engine.apply_sector(self._core, cin, op, out, alpha=alpha, beta=beta,

@@ -71,6 +71,8 @@ class RestrictedHamiltonian(Hamiltonian):
                  e_0: complex = 0.0 + 0.0j) -> None:
         super().__init__(e_0=e_0)
         self._tensor: Dict[int, numpy.ndarray] = {}
+        # device operators of the sector-by-sector apply, per device
+        self._prepared: Dict[Any, Any] = {}
         for rank, matrix in enumerate(tensors):
             if not (isinstance(rank, int) and
                     isinstance(matrix, numpy.ndarray)):
@@ -136,6 +138,8 @@ class _SpinOrbitalHamiltonian(Hamiltonian):
                  e_0: complex = 0.0 + 0.0j) -> None:
         super().__init__(e_0=e_0)
         self._tensor: Dict[int, numpy.ndarray] = {}
+        # device operators of the sector-by-sector apply, per device
+        self._prepared: Dict[Any, Any] = {}
         for rank, matrix in enumerate(tensors):
             if not isinstance(matrix, numpy.ndarray):
                 raise TypeError(
@@ -190,3 +194,10 @@ class GSOHamiltonian(_SpinOrbitalHamiltonian):
     """reference ``src/fqe/hamiltonians/gso_hamiltonian.py:23``"""
 
     _label = "GSOHamiltonian"
+
+
+class SSOHamiltonian(_SpinOrbitalHamiltonian):
+    """Distinct alpha and beta terms, no alpha/beta mixing blocks (reference
+    ``src/fqe/hamiltonians/sso_hamiltonian.py:26``)."""
+
+    _label = "SSOHamiltonian"

@@ -210,20 +210,23 @@ class Wavefunction:
         return out
 
     def _apply_spin_orbital(self, hamil) -> 'Wavefunction':
-        """H|psi> for spin-orbital tensors on a spin-broken wavefunction:
-        each particle number's FqeDataSet takes the apply, then e_0 psi is
-        added (reference ``src/fqe/wavefunction.py:349-357``, ``:366-399``).
-        Spin-conserving wavefunctions, whose sectors the reference applies
-        the spin-orbital tensors to one by one, are not supported here."""
+        """H|psi> for spin-orbital tensors (reference
+        ``src/fqe/wavefunction.py:349-357``, ``:366-399``): on a
+        spin-conserving wavefunction each sector takes the spin-orbital
+        apply of its spin-conserving blocks (``FqeData._apply_array_spin1``
+        / ``_apply_array_spin12``); on a spin-broken one each particle
+        number's FqeDataSet takes the apply; then e_0 psi is added."""
         expected = self._norb * 2
         if hamil.dim() != expected:
             raise ValueError('Hamiltonian has incorrect size:'
                              ' expected {}'.format(expected) +
                              ' provided {}'.format(hamil.dim()))
         if self._conserve_spin:
-            raise NotImplementedError(
-                "spin-orbital operators are applied to spin-broken "
-                "wavefunctions (broken=['spin']) through FqeDataSet")
+            if len(hamil.tensors()) > 2:
+                raise NotImplementedError(
+                    "the GPU engine applies 1- and 1+2-body operators; 3- "
+                    "and 4-body operators are outside its path")
+            return self._apply_array(hamil.tensors(), hamil.e_0(), hamil)
         out = copy.deepcopy(self)
         for nsector in out._number_sectors().values():
             nsector.apply_inplace(hamil.tensors())
@@ -283,7 +286,15 @@ class Wavefunction:
             if algo != 'taylor':
                 raise NotImplementedError(
                     "spin-orbital operators take the Taylor propagator")
-            out = _taylor_spin_orbital(self, time, hamil, accuracy, expansion)
+            if self._conserve_spin:
+                if len(hamil.tensors()) > 2:
+                    raise NotImplementedError(
+                        "the GPU engine applies 1- and 1+2-body operators")
+                # sector by sector, the whole series on the device
+                out = _taylor(self, time, hamil, accuracy, expansion)
+            else:
+                out = _taylor_spin_orbital(self, time, hamil, accuracy,
+                                           expansion)
         elif algo == 'chebyshev':
             from .chebyshev import chebyshev_evolve
             out = chebyshev_evolve(self, time, hamil, accuracy, expansion,
@@ -367,15 +378,18 @@ class Wavefunction:
                             format(b, 'b').zfill(self._norb), coeff[ia, ib]))
 
 
-def _prepared(hamil: Optional[RestrictedHamiltonian], array,
+def _prepared(hamil: Optional[Hamiltonian], array,
               norb: int) -> engine.PreparedOperator:
+    """The device operator of a restricted Hamiltonian, or of spin-orbital
+    tensors applied sector by sector (cached on the Hamiltonian)."""
     dev = _lib.device()
+    spin = numpy.asarray(array[0]).shape[0] == 2 * norb
+    prep = engine.prepare_spin_operator if spin else engine.prepare_operator
     if hamil is None:
-        return engine.prepare_operator(array, norb, dev)
-    key = ("op", dev.index)
+        return prep(array, norb, dev)
+    key = ("spin" if spin else "op", dev.index)
     if key not in hamil._prepared:
-        hamil._prepared[key] = engine.prepare_operator(hamil.tensors(), norb,
-                                                       dev)
+        hamil._prepared[key] = prep(hamil.tensors(), norb, dev)
     return hamil._prepared[key]
 
 

@@ -258,6 +258,111 @@ def apply_tensors(coeff, tensors, g: OracleGraph) -> numpy.ndarray:
     raise ValueError("oracle handles 1- and 2-body restricted tensors")
 
 
+# ---------------------------------------------------------------------------
+# spin-orbital tensors on one (nalpha, nbeta) sector
+# ---------------------------------------------------------------------------
+
+def dvec_spin(coeff: numpy.ndarray, g: OracleGraph):
+    """(Da, Db): Da[i,j] = E^alpha_ij C and Db[i,j] = E^beta_ij C (reference
+    ``src/fqe/fqe_data.py:2162-2175``, ``:2236-2285``)."""
+    n = g.norb
+    da = numpy.zeros((n, n, g.lena, g.lenb), dtype=numpy.complex128)
+    db = numpy.zeros_like(da)
+    for i in range(n):
+        for j in range(n):
+            am = g.alpha_map[(i, j)]
+            if am.size:
+                da[i, j, am[:, 1], :] += coeff[am[:, 0], :] * am[:, 2:3]
+            bm = g.beta_map[(i, j)]
+            if bm.size:
+                db[i, j, :, bm[:, 1]] += (coeff[:, bm[:, 0]] * bm[:, 2]).T
+    return da, db
+
+
+def coeff_from_dvec_spin(ya: numpy.ndarray, yb: numpy.ndarray,
+                         g: OracleGraph) -> numpy.ndarray:
+    """out = sum_ij E^alpha_ij Ya[i,j] + E^beta_ij Yb[i,j] (reference
+    ``src/fqe/fqe_data.py:2355-2383``, _calculate_coeff_spin_with_dvec:
+    maps taken as (j, i) and used from target to source, as in the
+    spatial case)."""
+    n = g.norb
+    out = numpy.zeros((g.lena, g.lenb), dtype=numpy.complex128)
+    for i in range(n):
+        for j in range(n):
+            am = g.alpha_map[(j, i)]
+            if am.size:
+                numpy.add.at(out, am[:, 0], ya[i, j, am[:, 1], :] * am[:, 2:3])
+            bm = g.beta_map[(j, i)]
+            if bm.size:
+                contrib = yb[i, j][:, bm[:, 1]] * bm[:, 2]
+                for c, s in enumerate(bm[:, 0]):
+                    out[:, s] += contrib[:, c]
+    return out
+
+
+def apply_spin1(coeff: numpy.ndarray, h1e: numpy.ndarray,
+                g: OracleGraph) -> numpy.ndarray:
+    """One-body spin-orbital tensor (2 norb x 2 norb, alpha first) on the
+    sector: only its alpha-alpha and beta-beta blocks act (reference
+    ``src/fqe/fqe_data.py:533-583``)."""
+    n = g.norb
+    h1e = numpy.asarray(h1e)
+    da, db = dvec_spin(numpy.asarray(coeff, dtype=numpy.complex128), g)
+    return (numpy.einsum("ij,ijkl->kl", h1e[:n, :n], da) +
+            numpy.einsum("ij,ijkl->kl", h1e[n:, n:], db))
+
+
+def apply_spin12(coeff: numpy.ndarray, h1e: numpy.ndarray, h2e: numpy.ndarray,
+                 g: OracleGraph) -> numpy.ndarray:
+    """One- and two-body spin-orbital tensors on the sector, the dvec form
+    with separate alpha and beta dvecs (reference
+    ``src/fqe/fqe_data.py:712-741``): V' = -moveaxis(h2e, 1, 2),
+    h1' = h1e - sum_k V'[i,k,k,j] over all 2 norb spin orbitals, then only
+    the spin-conserving blocks of h1' and V' act."""
+    n = g.norb
+    v = -numpy.moveaxis(numpy.asarray(h2e, dtype=numpy.complex128), 1, 2)
+    h1p = numpy.asarray(h1e, dtype=numpy.complex128).copy()
+    h1p -= numpy.einsum("ikkj->ij", v)
+    a, b = slice(0, n), slice(n, 2 * n)
+    da, db = dvec_spin(numpy.asarray(coeff, dtype=numpy.complex128), g)
+    out = (numpy.einsum("ij,ijkl->kl", h1p[a, a], da) +
+           numpy.einsum("ij,ijkl->kl", h1p[b, b], db))
+    ya = (numpy.einsum("ijkl,klmn->ijmn", v[a, a, a, a], da) +
+          numpy.einsum("ijkl,klmn->ijmn", v[a, a, b, b], db))
+    yb = (numpy.einsum("ijkl,klmn->ijmn", v[b, b, a, a], da) +
+          numpy.einsum("ijkl,klmn->ijmn", v[b, b, b, b], db))
+    out += coeff_from_dvec_spin(ya, yb, g)
+    return out
+
+
+def apply_spin_tensors(coeff, tensors, g: OracleGraph) -> numpy.ndarray:
+    """``FqeData.apply_inplace`` for 1- or 2-tensor spin-orbital tuples on
+    a spin-conserving sector (reference ``src/fqe/fqe_data.py:421-475``)."""
+    if len(tensors) == 1:
+        return apply_spin1(coeff, tensors[0], g)
+    if len(tensors) == 2:
+        return apply_spin12(coeff, tensors[0], tensors[1], g)
+    raise ValueError("oracle handles 1- and 2-body spin-orbital tensors")
+
+
+def restricted_as_spin_orbital(h1e: numpy.ndarray, h2e: numpy.ndarray):
+    """The spin-orbital tensors of a restricted (h1, h2): h1 on both spin
+    blocks and h2so[i rho, j eta, k rho, l eta] = h2[i, j, k, l], the
+    convention of h2[i,j,k,l] a+_i(rho) a+_j(eta) a_k(rho) a_l(eta)
+    (reference ``src/fqe/wavefunction.py:370-372``)."""
+    n = h1e.shape[0]
+    h1so = numpy.zeros((2 * n, 2 * n), dtype=numpy.result_type(h1e, h2e))
+    h1so[:n, :n] = h1e
+    h1so[n:, n:] = h1e
+    h2so = numpy.zeros((2 * n,) * 4, dtype=h1so.dtype)
+    for rho in range(2):
+        for eta in range(2):
+            r = slice(rho * n, (rho + 1) * n)
+            e = slice(eta * n, (eta + 1) * n)
+            h2so[r, e, r, e] = h2e
+    return h1so, h2so
+
+
 def apply_hamiltonian(coeff, tensors, e_0, g: OracleGraph) -> numpy.ndarray:
     """``Wavefunction._apply_array`` for one sector (reference
     ``src/fqe/wavefunction.py:366-399``): out = H C + e_0 C when

@@ -261,9 +261,19 @@ def test_gso_routing_errors(fqe):
     partial = fqe.Wavefunction([[2, 2, norb], [2, 0, norb]], broken=['spin'])
     with pytest.raises(ValueError, match="spin complete"):
         fqe.apply(ham, partial)
+    # on a spin-conserving wavefunction the sectors take the spin-orbital
+    # apply one by one, as in the reference: the identity h1 counts the
+    # electrons; 3-body tensors are outside the path
     conserving = fqe.Wavefunction([[2, 0, norb]])
+    conserving.set_wfn(strategy="random")
+    out = fqe.apply(ham, conserving)
+    assert numpy.allclose(out.get_coeff((2, 0)),
+                          2.0 * conserving.get_coeff((2, 0)), atol=1e-13)
     with pytest.raises(NotImplementedError):
-        fqe.apply(ham, conserving)
+        fqe.apply(fqe.get_gso_hamiltonian((numpy.zeros((m, m)),
+                                           numpy.zeros((m,) * 4),
+                                           numpy.zeros((m,) * 6))),
+                  conserving)
     full = fqe.Wavefunction([[2, 2, norb], [2, 0, norb], [2, -2, norb]],
                             broken=['spin'])
     with pytest.raises(ValueError, match="incorrect size"):
