@@ -36,7 +36,7 @@ int64_t needed_bytes(int precision, const fqehip_sector* s,
   if (fqehip_fused_supported(precision, s, op)) {
     int64_t b = static_cast<int64_t>(s->nslot + 1) * 2 * n * elem;
     if (precision == FQEHIP_C128 && s->hbb_f64 && s->tile == FQEHIP_TILE_OZAKI) {
-      b += s->lenb * 128 * 4;  // beta gather table
+      b += s->lenb * 128 * 4 + 128 * (5 * 128 + 12);  // beta table, pair digits
       b += s->lena * 8;  // row maxima of |C|
       if (s->hbb_oz)     // C digit planes and row scales
         b += 6 * 2 * s->lena * ((s->lenb + 63) / 64 * 64) + 8 * s->lena;

@@ -144,6 +144,86 @@ k_beta_table_oz(int npair, int64_t lenb, const int32_t* __restrict__ ptb,
   }
 }
 
+// Digits of one quarter row (32 consecutive k) of an operator row, by a
+// group of 4 consecutive lanes (one quarter each): the row's scale 2^ea with
+// |x 2^ea| < 2^38 from the group's maximum, the exact balanced digits of the
+// integers z = rint(x 2^ea) (from the low end, d = ((z + 128) mod 256) -
+// 128, z = (z - d) / 256; w[t] holds digit t, weight 2^(8(4 - t)), of k
+// 4x..4x+3 in word x), and the row sum of the z.
+__device__ __forceinline__ void quarter_row_digits(const double (&v)[32],
+                                                   uint32_t (&w)[kOzADig][8],
+                                                   int& ea, double& zsum) {
+  double amax = 0.0;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) amax = fmax(amax, fabs(v[j]));
+  amax = fmax(amax, __shfl_xor(amax, 1));
+  amax = fmax(amax, __shfl_xor(amax, 2));
+  ea = 0;
+  if (amax > 0.0) {
+    int e;
+    (void)frexp(amax, &e);
+    ea = 38 - e;
+  }
+  zsum = 0.0;
+#pragma unroll
+  for (int t = 0; t < kOzADig; ++t)
+#pragma unroll
+    for (int x = 0; x < 8; ++x) w[t][x] = 0u;
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    int64_t z = static_cast<int64_t>(rint(ldexp(v[j], ea)));
+    zsum += static_cast<double>(z);
+#pragma unroll
+    for (int t = kOzADig - 1; t >= 0; --t) {
+      const int64_t d = t ? ((z + 128) & 255) - 128 : z;
+      z = (z - d) >> 8;
+      w[t][j >> 2] |= (static_cast<uint32_t>(d) & 0xffu) << (8 * (j & 3));
+    }
+  }
+  zsum += __shfl_xor(zsum, 1);
+  zsum += __shfl_xor(zsum, 2);
+}
+
+// A' digits of every pair row p < npair (the same for every alpha string:
+// A[p][0..npair-1] and the h1' entry A[npair][p] at k = npair, zero beyond),
+// once per apply: pdig[p][t][k] bytes, pea[p] = ea, pzs[p] = sum of z.
+__global__ void __launch_bounds__(256)
+k_pair_digits_oz(int npair, const double* __restrict__ amat,
+                 uint8_t* __restrict__ pdig, int32_t* __restrict__ pea,
+                 double* __restrict__ pzs, const int32_t* __restrict__ done) {
+  if (done && *done) return;
+  const int p = blockIdx.x * 64 + (threadIdx.x >> 2);
+  const int k0 = (threadIdx.x & 3) * 32;
+  const int K = npair;
+  const bool ok = p < npair;
+  double v[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) {
+    const int k = k0 + j;
+    double x = 0.0;
+    if (ok) {
+      if (k < K) x = amat[static_cast<int64_t>(p) * K + k];
+      else if (k == K) x = amat[static_cast<int64_t>(npair) * K + p];
+    }
+    v[j] = x;
+  }
+  uint32_t w[kOzADig][8];
+  int ea;
+  double zsum;
+  quarter_row_digits(v, w, ea, zsum);
+  if (!ok) return;
+#pragma unroll
+  for (int t = 0; t < kOzADig; ++t) {
+    uint4* dst = reinterpret_cast<uint4*>(pdig + (static_cast<int64_t>(p) * kOzADig + t) * kOzKPad + k0);
+    dst[0] = make_uint4(w[t][0], w[t][1], w[t][2], w[t][3]);
+    dst[1] = make_uint4(w[t][4], w[t][5], w[t][6], w[t][7]);
+  }
+  if ((threadIdx.x & 3) == 0) {
+    pea[p] = ea;
+    pzs[p] = zsum;
+  }
+}
+
 // Alpha strings in an XCD-aware order: workgroup i runs on XCD i % 8, so
 // each XCD takes a contiguous range of strings, whose gather sources
 // overlap in its L2 (a bijection of [0, n)).
@@ -177,6 +257,8 @@ __global__ void __launch_bounds__(kRowThreads)
 k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
                 int64_t lenb, const double* __restrict__ amat,
                 const int32_t* __restrict__ pta, const uint32_t* __restrict__ btab,
+                const uint8_t* __restrict__ pdig, const int32_t* __restrict__ pea,
+                const double* __restrict__ pzs,
                 const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
                 int nslot, const double* __restrict__ coeff,
                 const double* __restrict__ rowmax, double* __restrict__ slots,
@@ -291,65 +373,63 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
       goff[q] = off >= 0 ? static_cast<uint32_t>(off) >> 1 : kNoRow16;
       gsa[q] = off >= 0 ? static_cast<double>(rt.asgn[q]) * dsc : 0.0;
     } else if (tid < 256) {
-      // A' digits: thread -> live row tid/4, k in [32 (tid%4), +32)
+      // live pair rows: A' digits of the apply's pair table (thread -> live
+      // row tid/4, k in [32 (tid%4), +32)); padding rows zero; the folded
+      // row is formed below
       const int m = tid >> 2;
       const int k0 = (tid & 3) * 32;
       const int p = lpair[m];
-      double v[32];
-      double amax = 0.0;
+      if (p != npair) {
+        uint4 w[kOzADig][2];
 #pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        const int k = k0 + j;
-        double x = 0.0;
-        if (p == npair) {
-          x = drow[k];
-        } else if (p >= 0) {
-          if (k < K) x = amat[static_cast<int64_t>(p) * K + k];
-          else if (k == K) x = amat[static_cast<int64_t>(npair) * K + p];
+        for (int t = 0; t < kOzADig; ++t) {
+          if (p >= 0) {
+            const uint4* src = reinterpret_cast<const uint4*>(
+                pdig + (static_cast<int64_t>(p) * kOzADig + t) * kOzKPad + k0);
+            w[t][0] = src[0];
+            w[t][1] = src[1];
+          } else {
+            w[t][0] = w[t][1] = make_uint4(0u, 0u, 0u, 0u);
+          }
         }
-        v[j] = x;
-        amax = fmax(amax, fabs(x));
-      }
-      amax = fmax(amax, __shfl_xor(amax, 1));
-      amax = fmax(amax, __shfl_xor(amax, 2));
-      int ea = 0;
-      if (amax > 0.0) {
-        int e;
-        (void)frexp(amax, &e);
-        ea = 38 - e;                        // |A' 2^ea| < 2^38
-      }
-      // exact balanced digits of the integer z = rint(x 2^ea), |z| < 2^38:
-      // from the low end, d = ((z + 128) mod 256) - 128, z = (z - d) / 256
-      double zsum = 0.0;
-      uint32_t w[kOzADig][8];
 #pragma unroll
-      for (int t = 0; t < kOzADig; ++t)
-#pragma unroll
-        for (int x = 0; x < 8; ++x) w[t][x] = 0u;
-#pragma unroll
-      for (int j = 0; j < 32; ++j) {
-        int64_t z = static_cast<int64_t>(rint(ldexp(v[j], ea)));
-        zsum += static_cast<double>(z);
-#pragma unroll
-        for (int t = kOzADig - 1; t >= 0; --t) {
-          const int64_t d = t ? ((z + 128) & 255) - 128 : z;
-          z = (z - d) >> 8;
-          w[t][j >> 2] |= (static_cast<uint32_t>(d) & 0xffu) << (8 * (j & 3));
+        for (int t = 0; t < kOzADig; ++t) {
+          uint4* dst = reinterpret_cast<uint4*>(adig + (t * kLiveRows + m) * kOzARow + k0);
+          dst[0] = w[t][0];
+          dst[1] = w[t][1];
+        }
+        if ((tid & 3) == 0) {
+          // Y = 2^(32 - ea - ed) t + 2^-(ea + ed) corr (powers of two: exact)
+          const double sc = p >= 0 ? ldexp(1.0, -(pea[p] + ed)) : 0.0;
+          corr[m] = p >= 0 ? kOffsetUnit * pzs[p] * sc : 0.0;
+          rscale[m] = 4294967296.0 * sc;
         }
       }
-      zsum += __shfl_xor(zsum, 1);
-      zsum += __shfl_xor(zsum, 2);
+    } else if (wave == 6) {
+      // the folded row (sum of the rows of ia's occupied diagonal pairs):
+      // lanes 4 g .. 4 g + 3 of group g < 4 take its quarters
+      const uint64_t fm = __ballot(lpair[lane] == npair);
+      const int m = fm ? __ffsll(static_cast<unsigned long long>(fm)) - 1 : -1;
+      if (m >= 0 && lane < 4) {
+        const int k0 = lane * 32;
+        double v[32];
 #pragma unroll
-      for (int t = 0; t < kOzADig; ++t) {
-        uint4* dst = reinterpret_cast<uint4*>(adig + (t * kLiveRows + m) * kOzARow + k0);
-        dst[0] = make_uint4(w[t][0], w[t][1], w[t][2], w[t][3]);
-        dst[1] = make_uint4(w[t][4], w[t][5], w[t][6], w[t][7]);
-      }
-      if ((tid & 3) == 0) {
-        // Y = 2^(32 - ea - ed) t + 2^-(ea + ed) corr (powers of two: exact)
-        const double sc = ldexp(1.0, -(ea + ed));
-        corr[m] = kOffsetUnit * zsum * sc;
-        rscale[m] = 4294967296.0 * sc;
+        for (int j = 0; j < 32; ++j) v[j] = drow[k0 + j];
+        uint32_t w[kOzADig][8];
+        int ea;
+        double zsum;
+        quarter_row_digits(v, w, ea, zsum);
+#pragma unroll
+        for (int t = 0; t < kOzADig; ++t) {
+          uint4* dst = reinterpret_cast<uint4*>(adig + (t * kLiveRows + m) * kOzARow + k0);
+          dst[0] = make_uint4(w[t][0], w[t][1], w[t][2], w[t][3]);
+          dst[1] = make_uint4(w[t][4], w[t][5], w[t][6], w[t][7]);
+        }
+        if (lane == 0) {
+          const double sc = ldexp(1.0, -(ea + ed));
+          corr[m] = kOffsetUnit * zsum * sc;
+          rscale[m] = 4294967296.0 * sc;
+        }
       }
     }
     __syncthreads();
@@ -448,10 +528,16 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
             double y[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-              double t = static_cast<double>(acc[c][5][e]);
-#pragma unroll
-              for (int m = 4; m >= 0; --m)
-                t = fma(t, 256.0, static_cast<double>(acc[c][m][e]));
+              // classes 5 and 4, and 3 and 2, merged in int32: with K <= 128,
+              // |acc5| <= 65 * 64 * 128 < 2^19.1, |acc4| < 2^21.1,
+              // |acc3| < 2^22.1, |acc2| < 2^22.7, so both merges stay below
+              // 2^31
+              const int32_t m54 = acc[c][5][e] * 256 + acc[c][4][e];
+              const int32_t m32 = acc[c][3][e] * 256 + acc[c][2][e];
+              double t = static_cast<double>(m54);
+              t = fma(t, 65536.0, static_cast<double>(m32));
+              t = fma(t, 256.0, static_cast<double>(acc[c][1][e]));
+              t = fma(t, 256.0, static_cast<double>(acc[c][0][e]));
               y[c] = fma(t, rscale[row], corr[row]);
             }
             const int po = lpair[row];
@@ -496,8 +582,13 @@ bool split_oz_supported(int norb, int npair, int64_t lena, int64_t lenb) {
          lena * lenb * 16 <= static_cast<int64_t>(kNoRow16) * 16;
 }
 
-// Bytes of the beta gather table (btab) of launch_split_oz.
-int64_t split_oz_table_bytes(int64_t lenb) { return lenb * kOzKPad * 4; }
+// Bytes of the tables of launch_split_oz: the beta gather table (lenb x
+// 128 words), then the pair rows' A' digits (128 x 5 x 128 bytes), their
+// scale exponents (128 int32) and integer sums (128 doubles).
+constexpr int64_t kPairDigitBytes = kOzKPad * (kOzADig * kOzKPad + 4 + 8);
+int64_t split_oz_table_bytes(int64_t lenb) {
+  return lenb * kOzKPad * 4 + kPairDigitBytes;
+}
 
 int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
@@ -513,6 +604,11 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
     return FQEHIP_ERR_INVALID;
   }
   uint32_t* tab = static_cast<uint32_t*>(btab);
+  uint8_t* pdig = reinterpret_cast<uint8_t*>(tab + lenb * kOzKPad);
+  int32_t* pea = reinterpret_cast<int32_t*>(pdig + kOzKPad * kOzADig * kOzKPad);
+  double* pzs = reinterpret_cast<double*>(pea + kOzKPad);
+  k_pair_digits_oz<<<(npair + 63) / 64, 256, 0, st>>>(npair, a, pdig, pea, pzs, done);
+  FQEHIP_CHECK_LAUNCH("k_pair_digits_oz");
   const bool row_lds = split_rows_oz_lds_bytes(true, lenb) <= kLdsLimit;
   const int64_t entries = lenb * kOzKPad;
   const unsigned tgrid = static_cast<unsigned>(std::min<int64_t>((entries + 255) / 256, 4096));
@@ -526,12 +622,12 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
   const size_t lds = split_rows_oz_lds_bytes(row_lds, lenb);
   if (row_lds)
     k_split_rows_oz<true><<<grid, kRowThreads, lds, st>>>(
-        norb, compressed, npair, lena, lenb, a, pta, tab, fwa, sla, nslot,
-        coeff, rowmax, slots, rowacc, done);
+        norb, compressed, npair, lena, lenb, a, pta, tab, pdig, pea, pzs, fwa,
+        sla, nslot, coeff, rowmax, slots, rowacc, done);
   else
     k_split_rows_oz<false><<<grid, kRowThreads, lds, st>>>(
-        norb, compressed, npair, lena, lenb, a, pta, tab, fwa, sla, nslot,
-        coeff, rowmax, slots, rowacc, done);
+        norb, compressed, npair, lena, lenb, a, pta, tab, pdig, pea, pzs, fwa,
+        sla, nslot, coeff, rowmax, slots, rowacc, done);
   FQEHIP_CHECK_LAUNCH("k_split_rows_oz");
   return FQEHIP_OK;
 }

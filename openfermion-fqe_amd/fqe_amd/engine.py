@@ -299,6 +299,7 @@ def workspace(sector: _lib.Sector, precision: int,
     fused = (sector.nslot + 1) * 2 * n * elem
     if sector.tile == _lib.TILE_OZAKI and sector.hbb_f64:
         fused += 512 * sector.lenb        # beta gather table
+        fused += 128 * (5 * 128 + 12)     # pair rows' digits
         fused += 8 * sector.lena          # row maxima of |C|
         if sector.hbb_oz:                 # C digit planes and row scales
             ldk = (sector.lenb + 63) // 64 * 64
