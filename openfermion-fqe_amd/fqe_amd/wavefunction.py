@@ -339,9 +339,30 @@ class Wavefunction:
 
     def read(self, filename: str, path: Optional[str] = None) -> None:
         """Initialise from a file written by ``save`` (reference
-        ``src/fqe/wavefunction.py:726-746``)."""
+        ``src/fqe/wavefunction.py:726-746``), or by the reference's own
+        ``save``: that file is a pickle, which is parsed without running
+        anything named in it (``reference_format``)."""
         full = os.path.join(path if path is not None else os.getcwd(),
                             filename)
+        with open(full, "rb") as f:
+            magic = f.read(2)
+        if magic[:1] == b"\x80":
+            from .reference_format import read_reference_wavefunction
+            with open(full, "rb") as f:
+                fields = read_reference_wavefunction(f.read())
+            self._norb = fields["norb"]
+            self._conserve_spin = fields["conserve_spin"]
+            self._conserve_number = fields["conserve_number"]
+            self._conserved = dict(fields["conserved"])
+            self._symmetry_map = dict(fields["symmetry_map"])
+            self._civec = {}
+            for (nele, m_s), coeff in fields["sectors"].items():
+                nalpha, nbeta = alpha_beta_electrons(nele, m_s)
+                self._dtype = numpy.dtype(coeff.dtype)
+                sec = FqeData(nalpha, nbeta, self._norb, dtype=self._dtype)
+                sec.coeff = coeff
+                self._civec[(nele, m_s)] = sec
+            return
         with numpy.load(full, allow_pickle=False) as z:
             header = json.loads(bytes(z["header"]).decode())
             if header.get("format") != self._FORMAT:
