@@ -10,10 +10,10 @@ batched propagator with a fixed step needs; ``run_sequential`` gives the
 same computation sector by sector without a graph (the FP64 comparison
 path).
 
-Kernel variants (``Variant``): the split apply (complex64, pair-symmetric
-operator: beta-beta GEMM + alpha row kernel), the row kernel with beta
-scatter at 4 or 8 waves per alpha row, and the unfused gather / GEMM /
-scatter kernels.  ``autotune`` times one apply of each candidate per
+Kernel variants (``Variant``): the split apply (pair-symmetric operator:
+beta-beta GEMM + alpha row kernel; in complex128 with FP64 or Ozaki-digit
+contractions, ``contraction``), the row kernel with beta scatter at 4 or 8
+waves per alpha row, and the unfused gather / GEMM / scatter kernels.  ``autotune`` times one apply of each candidate per
 distinct (norb, nalpha, nbeta) shape and keeps the fastest.
 """
 
@@ -45,15 +45,19 @@ class SectorSpec:
 
 
 def _struct_for(graph: FciGraph, op: engine.PreparedOperator, prec: int,
-                variant: str):
-    saved = settings.fused, settings.split, settings.split_fp64
+                variant: str, contraction: Optional[str] = None):
+    saved = (settings.fused, settings.split, settings.split_fp64,
+             settings.contraction)
     try:
         settings.fused = variant != "unfused"
         settings.split = variant == "split"
         settings.split_fp64 = variant == "split"
+        if contraction is not None:
+            settings.contraction = contraction
         s, keep = engine.sector_struct(graph, op, prec)
     finally:
-        settings.fused, settings.split, settings.split_fp64 = saved
+        (settings.fused, settings.split, settings.split_fp64,
+         settings.contraction) = saved
     if variant == "rows8":
         s.tile = 8
     elif variant == "rows4":
@@ -79,7 +83,8 @@ class _Prepared:
     """Device state of one sector of a batch."""
 
     def __init__(self, spec: SectorSpec, dtype: torch.dtype,
-                 variant: Optional[str]) -> None:
+                 variant: Optional[str],
+                 contraction: Optional[str] = None) -> None:
         dev = _lib.device()
         self.spec = spec
         self.graph = FciGraph(spec.nalpha, spec.nbeta, spec.norb)
@@ -87,7 +92,7 @@ class _Prepared:
         self.op = engine.prepare_operator((spec.h1, spec.h2), spec.norb, dev)
         self.variant = variant or variants_for(self.op, self.prec)[0]
         self.s, self.keep = _struct_for(self.graph, self.op, self.prec,
-                                        self.variant)
+                                        self.variant, contraction)
         self.opc = self.op.c_struct()
         shape = (self.graph.lena(), self.graph.lenb())
         self.c0 = torch.from_numpy(numpy.asarray(spec.coeff).reshape(shape)
@@ -107,11 +112,13 @@ class _Prepared:
 
 
 def autotune(specs: Sequence[SectorSpec], dtype: torch.dtype = torch.complex64,
-             reps: int = 3, cache: Optional[Dict[str, str]] = None
-             ) -> Dict[str, str]:
+             reps: int = 3, cache: Optional[Dict[str, str]] = None,
+             contraction: Optional[str] = None) -> Dict[str, str]:
     """Fastest variant per (norb, nalpha, nbeta) shape: one apply of the
     first sector of each shape per candidate, median of ``reps`` timed
-    launches after one warm-up.  ``cache`` entries are kept as they are."""
+    launches after one warm-up.  ``cache`` entries are kept as they are.
+    ``contraction`` (complex128): the arithmetic of the split variant,
+    "fp64" or "ozaki" (default: ``settings.contraction``)."""
     table = dict(cache or {})
     seen = {}
     for sp in specs:
@@ -120,9 +127,10 @@ def autotune(specs: Sequence[SectorSpec], dtype: torch.dtype = torch.complex64,
             seen[key] = sp
     for key, sp in seen.items():
         best, best_t = None, None
-        base = _Prepared(sp, dtype, None)
+        base = _Prepared(sp, dtype, None, contraction)
         for var in variants_for(base.op, base.prec):
-            p = base if var == base.variant else _Prepared(sp, dtype, var)
+            p = (base if var == base.variant else
+                 _Prepared(sp, dtype, var, contraction))
             times = []
             for r in range(reps + 1):
                 ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
@@ -151,14 +159,16 @@ class BatchedTaylor:
     def __init__(self, specs: Sequence[SectorSpec], time_step: float,
                  nterms: int = 10, dtype: torch.dtype = torch.complex64,
                  tiles: Optional[Dict[str, str]] = None,
-                 nstreams: int = 8, group: bool = True) -> None:
+                 nstreams: int = 8, group: bool = True,
+                 contraction: Optional[str] = None) -> None:
         self.time = float(time_step)
         self.nterms = int(nterms)
         self.dtype = dtype
         self.tiles = tiles or {}
         self.sectors = [
             _Prepared(sp, dtype,
-                      self.tiles.get(f"{sp.norb},{sp.nalpha},{sp.nbeta}"))
+                      self.tiles.get(f"{sp.norb},{sp.nalpha},{sp.nbeta}"),
+                      contraction)
             for sp in specs]
         self.prec = engine.precision_of(dtype)
         self.nstreams = nstreams

@@ -6,7 +6,9 @@
   norb <= 8 sectors against the oracle to 1e-12, and the complex64 graph
   against the FP64 path within its stated bounds (apply 1e-4, Taylor
   1e-3: BF16x3 products with FP32 accumulation, errors of about 5e-6 per
-  apply compounding over ten terms).
+  apply compounding over ten terms), and the complex128 graph with every
+  split-capable sector on the Ozaki-digit kernels against the FP64 path
+  to 1e-9.
 * Davidson at norb=14 7a7b (held-out seed-4 Hamiltonian, FP64 applies):
   residual 1e-8, recomputed from a fresh apply; the eigenvalue of a second
   solve from another start vector agrees to 1e-9.
@@ -88,6 +90,23 @@ def test_batched_256_sectors_seed3(fqe):
     print(f"  complex64 graph vs FP64: apply {ea:.2e}, Taylor {ee:.2e}",
           flush=True)
     assert ea < 1e-4 and ee < 1e-3
+    # the mixed engine at <= 1e-9: complex128 graph, every split-capable
+    # sector on the Ozaki-digit split kernels
+    from fqe_amd import _lib
+    tiles = {f"{sp.norb},{sp.nalpha},{sp.nbeta}": "split" for sp in specs}
+    oz = batched.BatchedTaylor(specs, 0.1, nterms=10, dtype=torch.complex128,
+                               nstreams=8, tiles=tiles, contraction="ozaki")
+    ndigit = sum(1 for p in oz.sectors if p.s.tile == _lib.TILE_OZAKI)
+    oz.build()
+    oz.launch()
+    ores = oz.results()
+    oz.close()
+    ea = max(rel(a, b[0]) for (a, _), b in zip(ores, seq))
+    ee = max(rel(e, b[1]) for (_, e), b in zip(ores, seq))
+    print(f"  Ozaki graph ({ndigit} digit sectors) vs FP64: apply {ea:.2e}, "
+          f"Taylor {ee:.2e}", flush=True)
+    assert ndigit > 100
+    assert ea < 1e-9 and ee < 1e-9
 
 
 def test_davidson_norb14_seed4(fqe):
