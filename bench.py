@@ -1001,6 +1001,9 @@ def main():
                "ozaki": (torch.complex128, "ozaki"),
                "bf16x3": (torch.complex64, "fp64")}
     modes = [m for m in engines if m in args.modes.split(",")]
+    if args.unfused:
+        # the Ozaki-digit contractions exist only in the split kernels
+        modes = [m for m in modes if m != "ozaki"]
     results, outs = {}, {}
     for mode in modes:
         dt, contraction = engines[mode]
@@ -1018,10 +1021,19 @@ def main():
         del run
     if set(modes) != set(engines):
         for mode in modes:
-            print(json.dumps({"mode": mode,
-                              "applies_per_s": args.steps /
-                              results[mode]["wall"],
-                              "kernels_ms": results[mode]["per"]}))
+            line = {"mode": mode,
+                    "applies_per_s": args.steps / results[mode]["wall"],
+                    "kernels_ms": results[mode]["per"]}
+            if not results[mode]["fused"]:
+                # the (b) gather, (c) pair GEMM and (d) scatter against
+                # their own rooflines
+                esz = 8 if mode == "fp64" else 4
+                work = kernel_work(op, graph, n, esz, mode != "fp64")
+                line["rooflines"] = [
+                    roofline_entry(k, results[mode]["per"], work,
+                                   "fp64" if mode == "fp64" else "mixed")
+                    for k in ("dvec_gather", "pair_gemm", "scatter")]
+            print(json.dumps(line))
         return
 
     def rel(a, b):

@@ -81,6 +81,67 @@ k_dvec_gather(int64_t lena, int64_t lenb, double inv_lenb,
   }
 }
 
+// Row form of the same gather (the one the apply uses): one workgroup per
+// alpha string ia writes D[p][re|im][ia, :] for every plane p.
+//   - C[ia, :] is staged once in LDS (ROW_LDS; else read from the cache),
+//     so the beta term sum over ptb[p][ib] = sign (src + 1) is an LDS
+//     gather;
+//   - the alpha term is the whole row C[src, :] of the alpha source of
+//     (p, ia), one sign for the workgroup: a coalesced 16-byte read per lane
+//     (complex128), served by the caches for the second and later targets
+//     that share a source;
+//   - the planes are written with non-temporal stores, coalesced along ib.
+constexpr int kRowBlock = 512;
+constexpr int64_t kRowLdsBytes = 64 * 1024;
+
+template <typename T, bool ROW_LDS>
+__global__ void __launch_bounds__(kRowBlock)
+k_dvec_rows(int npair, int64_t lena, int64_t lenb,
+            const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+            const T* __restrict__ coeff, T* __restrict__ dvec,
+            const int32_t* __restrict__ done) {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  extern __shared__ __align__(16) unsigned char row_raw[];
+  T2* row = reinterpret_cast<T2*>(row_raw);
+  if (done && *done) return;
+  const int64_t ia = blockIdx.x;
+  const int64_t n = lena * lenb;
+  const T2* c2 = reinterpret_cast<const T2*>(coeff);
+  const T2* own = c2 + ia * lenb;
+  if (ROW_LDS) {
+    for (int64_t ib = threadIdx.x; ib < lenb; ib += kRowBlock) row[ib] = own[ib];
+    __syncthreads();
+  }
+  const int64_t base = ia * lenb;
+  for (int p = 0; p < npair; ++p) {
+    // uniform over the workgroup
+    const int32_t a = __builtin_amdgcn_readfirstlane(pta[static_cast<int64_t>(p) * lena + ia]);
+    const T2* src = c2 + static_cast<int64_t>((a > 0 ? a : -a) - 1) * lenb;
+    const T sa = a > 0 ? T(1) : T(-1);
+    const int32_t* pb = ptb + static_cast<int64_t>(p) * lenb;
+    T* dre = dvec + static_cast<int64_t>(2 * p) * n + base;
+    T* dim = dre + n;
+#pragma unroll 2
+    for (int64_t ib = threadIdx.x; ib < lenb; ib += kRowBlock) {
+      T re = 0, im = 0;
+      if (a) {
+        const T2 v = src[ib];
+        re = sa * v[0];
+        im = sa * v[1];
+      }
+      const int32_t b = pb[ib];
+      if (b) {
+        const int64_t j = (b > 0 ? b : -b) - 1;
+        const T2 w = ROW_LDS ? row[j] : own[j];
+        re += b > 0 ? w[0] : -w[0];
+        im += b > 0 ? w[1] : -w[1];
+      }
+      __builtin_nontemporal_store(re, dre + ib);
+      __builtin_nontemporal_store(im, dim + ib);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // (c) FP64 GEMM Y = A B on v_mfma_f64_16x16x4f64.
 // A: m x k (row-major, read through the cache; it is tiny and shared by all
@@ -378,6 +439,17 @@ __global__ void k_status_init(int32_t* status) {
   }
 }
 
+}  // namespace
+
+// Defined in pair_gemm.hip.
+bool pair_gemm_lds_ok(int m, int k, int64_t ncol, const void* b,
+                      const void* y);
+int launch_pair_gemm_lds(int m, int k, const double* a, int lda, int64_t ncol,
+                         const double* b, double* y, int accumulate,
+                         hipStream_t st, const int32_t* done);
+
+namespace {
+
 unsigned grid_stride_blocks(int64_t n) {
   const unsigned b = blocks_for(n, kBlock);
   return b < 65536u ? (b == 0 ? 1u : b) : 65536u;
@@ -395,6 +467,19 @@ int launch_gather(int npair, int64_t lena, int64_t lenb, const int32_t* pta,
                   hipStream_t st, const int32_t* done) {
   const int64_t n = lena * lenb;
   if (n == 0 || npair == 0) return FQEHIP_OK;
+  if (lena <= 0x7fffffff) {
+    const int64_t row = lenb * 2 * static_cast<int64_t>(sizeof(T));
+    if (row <= kRowLdsBytes)
+      k_dvec_rows<T, true><<<static_cast<unsigned>(lena), kRowBlock, row, st>>>(
+          npair, lena, lenb, pta, ptb, static_cast<const T*>(coeff),
+          static_cast<T*>(dvec), done);
+    else
+      k_dvec_rows<T, false><<<static_cast<unsigned>(lena), kRowBlock, 0, st>>>(
+          npair, lena, lenb, pta, ptb, static_cast<const T*>(coeff),
+          static_cast<T*>(dvec), done);
+    FQEHIP_CHECK_LAUNCH("k_dvec_rows");
+    return FQEHIP_OK;
+  }
   unsigned bx = blocks_for(n, kBlock);
   if (bx > 8192u) bx = 8192u;
   dim3 grid(bx, npair);
@@ -441,6 +526,9 @@ int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
                     const double* b, double* y, hipStream_t st,
                     const int32_t* done, int lda = -1, int accumulate = 0) {
   if (m == 0 || ncol == 0) return FQEHIP_OK;
+  if (pair_gemm_lds_ok(m, k, ncol, b, y))
+    return launch_pair_gemm_lds(m, k, a, lda < 0 ? k : lda, ncol, b, y,
+                                accumulate, st, done);
   dim3 grid(blocks_for(ncol, 64), (m + kGemmRowTiles * 16 - 1) / (kGemmRowTiles * 16));
   k_gemm_f64<<<grid, kBlock, 0, st>>>(m, k, a, lda < 0 ? k : lda, ncol, b, y,
                                       accumulate, done);
