@@ -86,6 +86,9 @@ def parse():
                          "kernel also scatters beta excitations)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the unfused gather / GEMM / scatter kernels")
+    ap.add_argument("--complex-op", action="store_true",
+                    help="give h1 and h2 imaginary parts (complex operator: "
+                         "full pair basis, real 2 x 2 block GEMM)")
     ap.add_argument("--batched", action="store_true",
                     help="run the batched-sector workload instead (256 "
                          "sectors, norb 6..12, seed 3, one apply + 10-term "
@@ -992,6 +995,11 @@ def main():
         del src, dst
     norb, na, nb = args.norb, args.nalpha, args.nbeta
     coeff, h1, h2 = synthetic.workload(norb, na, nb, args.seed, norb)
+    if args.complex_op:
+        rng = numpy.random.default_rng(args.seed + 1000)
+        h1 = h1 + 0.05j * rng.normal(size=h1.shape)
+        h1 = 0.5 * (h1 + h1.conj().T)
+        h2 = h2 + 0.01j * rng.normal(size=h2.shape)
     graph = fqe.FciGraph(na, nb, norb)
     dev = _lib.device()
     op = engine.prepare_operator((h1, h2), norb, dev)
