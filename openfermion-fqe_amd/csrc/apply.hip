@@ -41,6 +41,21 @@ __device__ __forceinline__ T signed_val(int32_t code, T v) {
   return code > 0 ? v : -v;
 }
 
+template <typename T> struct vec2_of;
+template <> struct vec2_of<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+template <> struct vec2_of<float> {
+  typedef float type __attribute__((ext_vector_type(2)));
+};
+template <typename T> struct vec4_of;
+template <> struct vec4_of<double> {
+  typedef double type __attribute__((ext_vector_type(4)));
+};
+template <> struct vec4_of<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+
 // ---------------------------------------------------------------------------
 // (b) dvec gather: D[p][re|im][I] for one plane p per blockIdx.y.
 // ---------------------------------------------------------------------------
@@ -87,12 +102,45 @@ k_dvec_gather(int64_t lena, int64_t lenb, double inv_lenb,
 //     so the beta term sum over ptb[p][ib] = sign (src + 1) is an LDS
 //     gather;
 //   - the alpha term is the whole row C[src, :] of the alpha source of
-//     (p, ia), one sign for the workgroup: a coalesced 16-byte read per lane
-//     (complex128), served by the caches for the second and later targets
-//     that share a source;
-//   - the planes are written with non-temporal stores, coalesced along ib.
+//     (p, ia), one sign for the workgroup: coalesced reads, served by the
+//     caches for the second and later targets that share a source;
+//   - the planes are written with non-temporal stores, coalesced along ib;
+//   - with lenb even a lane takes a column pair (one 32- or 16-byte read
+//     of C, one 8-byte read of the table, two plane stores of 2 values).
 constexpr int kRowBlock = 512;
 constexpr int64_t kRowLdsBytes = 64 * 1024;
+
+// One column pair (ib, ib + 1) of plane p for target row ia: alpha term
+// from the source row src (null when the table entry is 0), beta term
+// from the LDS row (or the global row own).
+template <typename T, bool ROW_LDS>
+__device__ __forceinline__ void dvec_pair(
+    const T* __restrict__ src, T sa,
+    const typename vec2_of<T>::type* __restrict__ row,
+    const typename vec2_of<T>::type* __restrict__ own, int2 b, int64_t ib,
+    typename vec2_of<T>::type& re, typename vec2_of<T>::type& im) {
+  typedef typename vec2_of<T>::type T2;
+  typedef typename vec4_of<T>::type T4;
+  re = T2{0, 0};
+  im = T2{0, 0};
+  if (src) {
+    const T4 v = *reinterpret_cast<const T4*>(src + 2 * ib);
+    re = T2{sa * v[0], sa * v[2]};
+    im = T2{sa * v[1], sa * v[3]};
+  }
+  if (b.x) {
+    const int64_t j = (b.x > 0 ? b.x : -b.x) - 1;
+    const T2 w = ROW_LDS ? row[j] : own[j];
+    re[0] += b.x > 0 ? w[0] : -w[0];
+    im[0] += b.x > 0 ? w[1] : -w[1];
+  }
+  if (b.y) {
+    const int64_t j = (b.y > 0 ? b.y : -b.y) - 1;
+    const T2 w = ROW_LDS ? row[j] : own[j];
+    re[1] += b.y > 0 ? w[0] : -w[0];
+    im[1] += b.y > 0 ? w[1] : -w[1];
+  }
+}
 
 template <typename T, bool ROW_LDS>
 __global__ void __launch_bounds__(kRowBlock)
@@ -100,7 +148,7 @@ k_dvec_rows(int npair, int64_t lena, int64_t lenb,
             const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
             const T* __restrict__ coeff, T* __restrict__ dvec,
             const int32_t* __restrict__ done) {
-  typedef T T2 __attribute__((ext_vector_type(2)));
+  typedef typename vec2_of<T>::type T2;
   extern __shared__ __align__(16) unsigned char row_raw[];
   T2* row = reinterpret_cast<T2*>(row_raw);
   if (done && *done) return;
@@ -113,21 +161,33 @@ k_dvec_rows(int npair, int64_t lena, int64_t lenb,
     __syncthreads();
   }
   const int64_t base = ia * lenb;
+  // lenb even: every row starts on an even element, and each lane takes a
+  // column pair (16-byte plane stores in complex128, 8-byte in complex64)
+  const bool pairs = (lenb & 1) == 0;
   for (int p = 0; p < npair; ++p) {
     // uniform over the workgroup
     const int32_t a = __builtin_amdgcn_readfirstlane(pta[static_cast<int64_t>(p) * lena + ia]);
-    const T2* src = c2 + static_cast<int64_t>((a > 0 ? a : -a) - 1) * lenb;
+    const T* src = a ? coeff + 2 * static_cast<int64_t>((a > 0 ? a : -a) - 1) * lenb
+                     : nullptr;
     const T sa = a > 0 ? T(1) : T(-1);
     const int32_t* pb = ptb + static_cast<int64_t>(p) * lenb;
     T* dre = dvec + static_cast<int64_t>(2 * p) * n + base;
     T* dim = dre + n;
-#pragma unroll 2
+    if (pairs) {
+      for (int64_t ib = 2 * threadIdx.x; ib < lenb; ib += 2 * kRowBlock) {
+        T2 re, im;
+        dvec_pair<T, ROW_LDS>(src, sa, row, own,
+                              *reinterpret_cast<const int2*>(pb + ib), ib, re, im);
+        __builtin_nontemporal_store(re, reinterpret_cast<T2*>(dre + ib));
+        __builtin_nontemporal_store(im, reinterpret_cast<T2*>(dim + ib));
+      }
+      continue;
+    }
     for (int64_t ib = threadIdx.x; ib < lenb; ib += kRowBlock) {
       T re = 0, im = 0;
-      if (a) {
-        const T2 v = src[ib];
-        re = sa * v[0];
-        im = sa * v[1];
+      if (src) {
+        re = sa * src[2 * ib];
+        im = sa * src[2 * ib + 1];
       }
       const int32_t b = pb[ib];
       if (b) {
