@@ -152,7 +152,8 @@ k_dvec_rows(int npair, int64_t lena, int64_t lenb,
   extern __shared__ __align__(16) unsigned char row_raw[];
   T2* row = reinterpret_cast<T2*>(row_raw);
   if (done && *done) return;
-  const int64_t ia = blockIdx.x;
+  // targets of one XCD are a contiguous range: they share alpha sources
+  const int64_t ia = xcd_order(blockIdx.x, lena);
   const int64_t n = lena * lenb;
   const T2* c2 = reinterpret_cast<const T2*>(coeff);
   const T2* own = c2 + ia * lenb;

@@ -36,6 +36,17 @@ void clear_error();
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// XCD-aware order of n work items (alpha strings, element blocks):
+// workgroup i runs on XCD i % 8 of an MI355X, so XCD x takes the
+// contiguous range of items starting at x n/8 (+1 per earlier XCD with a
+// remainder item); items that read the same rows then share one L2.  A
+// bijection of [0, n).
+__device__ __forceinline__ int64_t xcd_order(int64_t i, int64_t n) {
+  const int64_t per = n / 8, rem = n % 8;
+  const int64_t x = i % 8, j = i / 8;
+  return x * per + (x < rem ? x : rem) + j;
+}
+
 // Grid-size helper: number of blocks of `block` threads covering n items.
 inline unsigned blocks_for(int64_t n, int block) {
   return static_cast<unsigned>((n + block - 1) / block);

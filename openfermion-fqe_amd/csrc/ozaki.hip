@@ -224,15 +224,6 @@ k_pair_digits_oz(int npair, const double* __restrict__ amat,
   }
 }
 
-// Alpha strings in an XCD-aware order: workgroup i runs on XCD i % 8, so
-// each XCD takes a contiguous range of strings, whose gather sources
-// overlap in its L2 (a bijection of [0, n)).
-__device__ __forceinline__ int64_t xcd_order(int64_t i, int64_t n) {
-  const int64_t per = n / 8, rem = n % 8;
-  const int64_t x = i % 8, j = i / 8;
-  return x * per + (x < rem ? x : rem) + j;
-}
-
 constexpr int kRowThreads = 512;           // 8 waves, 16 columns each
 constexpr int kRowCols = 128;              // columns per chunk
 
