@@ -465,7 +465,9 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
     from fqe_amd import settings
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
-    flops = 2.0 * (p + 1) * p * 2 * n
+    # a complex operator runs on its real 2 x 2 block form: m = 2(p + 1),
+    # k = 2p on n columns, twice the flops of the real (p + 1) x p on 2n
+    flops = 2.0 * (p + 1) * p * 2 * n * (2 if op.complex_op else 1)
     rows = p + 1 if (mixed and settings.split_tile) else nslot + 1
     lenb = graph.lenb()
     ldh = ((lenb + 127) // 128) * 128
