@@ -523,9 +523,10 @@ typedef struct fqehip_batch fqehip_batch;
  * (1..32), on capture streams of the batch's own (`stream` is not used
  * while recording).  Buffers are referenced, not copied: every launch on
  * a stream reads d_coeff0 and rewrites the outputs.  With group != 0 the
- * complex64 sectors that take the fused row kernel without the split
- * (hbb_hi NULL) run as one group: per Taylor step one row launch and one
- * reduction cover all of them (the rows4 row body), instead of kernels per
+ * sectors that take the fused row kernel without the split (hbb_hi and
+ * hbb_f64 NULL, tile not FQEHIP_TILE_OZAKI) run as one group: per Taylor
+ * step one row launch and one reduction cover all of them (complex64: the
+ * rows4 row body; complex128: the FP64 row body), instead of kernels per
  * sector; the other sectors keep their own kernels on the other branches. */
 int fqehip_batch_create(int precision, int nitems,
                         const fqehip_batch_item* items, double time,

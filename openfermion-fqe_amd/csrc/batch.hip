@@ -26,10 +26,12 @@ struct fqehip_batch {
 namespace fqehip {
 
 // Defined in fused.hip.
-int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
-                      int64_t max_lenb, int step, hipStream_t st);
-int launch_group_reduce(const GroupSector* d_secs, int nsec, int64_t blocks,
-                        int step, double ar, double ai, hipStream_t st);
+int launch_group_rows(int precision, const GroupSector* d_secs, int nsec,
+                      int64_t rows, int64_t max_lenb, int step,
+                      hipStream_t st);
+int launch_group_reduce(int precision, const GroupSector* d_secs, int nsec,
+                        int64_t blocks, int step, double ar, double ai,
+                        hipStream_t st);
 int group_reduce_threads();
 
 namespace {
@@ -112,9 +114,10 @@ int enqueue_group(int precision, const fqehip_batch_item* items,
   }
   const int nsec = static_cast<int>(grouped.size());
   for (int step = any_apply ? 0 : 1; step <= nterms; ++step) {
-    int rc = launch_group_rows(d_secs, nsec, rows, max_lenb, step, st);
+    int rc = launch_group_rows(precision, d_secs, nsec, rows, max_lenb, step,
+                               st);
     if (rc) return rc;
-    rc = launch_group_reduce(d_secs, nsec, blocks, step, 0.0,
+    rc = launch_group_reduce(precision, d_secs, nsec, blocks, step, 0.0,
                              step ? -time / static_cast<double>(step) : 0.0, st);
     if (rc) return rc;
     *nodes += 2;
@@ -178,13 +181,15 @@ extern "C" int fqehip_batch_create(int precision, int nitems,
     b->events.push_back(ev);
   }
   hipStream_t origin = b->streams[nb];
-  // complex64 sectors of the rows4/rows8 kinds (fused, no split) run as
-  // one group: one row launch and one reduction per step for all of them
+  // sectors of the fused row kinds (no split: complex64 rows4/rows8,
+  // complex128 k_fused_f64) run as one group: one row launch and one
+  // reduction per step for all of them
   std::vector<int> grouped, single;
   for (int i = 0; i < nitems; ++i) {
     const fqehip_batch_item& it = items[i];
-    const bool g = group && precision == FQEHIP_C64 &&
-                   it.sector.lena * it.sector.lenb > 0 && !it.sector.hbb_hi &&
+    const bool g = group && it.sector.lena * it.sector.lenb > 0 &&
+                   !it.sector.hbb_hi && !it.sector.hbb_f64 &&
+                   it.sector.tile != FQEHIP_TILE_OZAKI &&
                    it.sector.nslot == it.sector.nalpha * (it.sector.norb - it.sector.nalpha) &&
                    fqehip_fused_supported(precision, &it.sector, &it.op);
     (g ? grouped : single).push_back(i);
@@ -214,12 +219,13 @@ extern "C" int fqehip_batch_create(int precision, int nitems,
       g.fwa = sc.fwd_a;
       g.fwb = sc.fwd_b;
       g.sla = sc.slot_a;
-      g.coeff0 = static_cast<const float*>(it.d_coeff0);
-      g.apply_out = static_cast<float*>(it.d_apply_out);
-      g.term = static_cast<float*>(it.d_term);
+      g.coeff0 = it.d_coeff0;
+      g.apply_out = it.d_apply_out;
+      g.term = it.d_term;
       g.result = it.d_result;
-      g.slots = static_cast<float*>(it.d_workspace);
-      g.rowacc = g.slots + static_cast<int64_t>(sc.nslot) * 2 * n;
+      g.slots = it.d_workspace;
+      g.rowacc = static_cast<char*>(it.d_workspace) +
+                 static_cast<int64_t>(sc.nslot) * n * esize(precision);
       grows += sc.lena;
       gblocks += (n + nt - 1) / nt;
       gmax_lenb = std::max(gmax_lenb, sc.lenb);

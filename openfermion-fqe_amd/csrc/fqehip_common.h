@@ -130,12 +130,13 @@ struct GroupSector {
   const int32_t* fwa;
   const int32_t* fwb;
   const int32_t* sla;
-  const float* coeff0;          // step 0 input
-  float* apply_out;             // step 0 output; NULL: the sector has no step 0
-  float* term;                  // steps >= 1: input and output (in place)
+  // complex64 (float) or complex128 (double) arrays, by the group's precision
+  const void* coeff0;           // step 0 input
+  void* apply_out;              // step 0 output; NULL: the sector has no step 0
+  void* term;                   // steps >= 1: input and output (in place)
   double* result;               // steps >= 1: FP64 Taylor sum
-  float* slots;                 // workspace: slot rows
-  float* rowacc;                // workspace: row accumulators
+  void* slots;                  // workspace: slot rows
+  void* rowacc;                 // workspace: row accumulators
 };
 
 }  // namespace fqehip

@@ -661,21 +661,37 @@ constexpr int kF64Groups = kFThreads / kF64TbC;
 constexpr int kF64GIter = kF64KPad / kF64Groups;
 constexpr int kF64Tile = kF64KPad * kF64Stride;
 
-__global__ void __launch_bounds__(kFThreads, 2)
-k_fused_f64(int npair, int64_t lena, int64_t lenb,
-            const double* __restrict__ amat, const int32_t* __restrict__ pta,
-            const int32_t* __restrict__ ptb, const int32_t* __restrict__ fwa,
-            const int32_t* __restrict__ fwb, const int32_t* __restrict__ sla,
-            int nslot, const double* __restrict__ coeff,
-            double* __restrict__ slots, double* __restrict__ rowacc,
-            const int32_t* __restrict__ done) {
-  extern __shared__ __align__(16) char smem[];
-  if (done && *done) return;
+// A' fragments of the wave's 16 rows in MFMA A-operand registers (lane l:
+// row 16 wave + l%16, k = 4 kt + l/16), kept for every alpha row.
+__device__ __forceinline__ void f64_a_regs(int npair,
+                                           const double* __restrict__ amat,
+                                           double (&areg)[kF64KSteps]) {
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int K = npair;
+  const int M = npair + 1;
+  const int row = wave * 16 + (lane & 15);
+#pragma unroll
+  for (int kt = 0; kt < kF64KSteps; ++kt) {
+    const int kk = kt * 4 + (lane >> 4);
+    areg[kt] = (row < M && kk < K) ? amat[static_cast<int64_t>(row) * K + kk] : 0.0;
+  }
+}
+
+// Alpha row ia of k_fused_f64 (shared with the grouped launch over the
+// sectors of a batch, k_fused_group_f64).
+__device__ __forceinline__ void f64_row(
+    int64_t ia, int npair, int64_t lena, int64_t lenb,
+    const int32_t* __restrict__ pta, const int32_t* __restrict__ ptb,
+    const int32_t* __restrict__ fwa, const int32_t* __restrict__ fwb,
+    const int32_t* __restrict__ sla, int nslot,
+    const double* __restrict__ coeff, double* __restrict__ slots,
+    double* __restrict__ rowacc, char* smem,
+    const double (&areg)[kF64KSteps]) {
   const int tid = threadIdx.x;
   const int wave = tid >> 6;
   const int lane = tid & 63;
   const int K = npair;
-  const int M = npair + 1;
   const int ksteps = (K + 3) / 4;
   double* dt = reinterpret_cast<double*>(smem);     // 2 x [kF64KPad][kF64Stride]
   RowTables rt;
@@ -685,15 +701,6 @@ k_fused_f64(int npair, int64_t lena, int64_t lenb,
   rt.ssgn = rt.asgn + kRowTabs;
   double* accrow = reinterpret_cast<double*>(rt.ssgn + kRowTabs);
 
-  double areg[kF64KSteps];
-  {
-    const int row = wave * 16 + (lane & 15);
-#pragma unroll
-    for (int kt = 0; kt < kF64KSteps; ++kt) {
-      const int kk = kt * 4 + (lane >> 4);
-      areg[kt] = (row < M && kk < K) ? amat[static_cast<int64_t>(row) * K + kk] : 0.0;
-    }
-  }
   const bool wave_live = wave * 16 <= npair;
   const int t = tid % kF64TbC;
   const int g = tid / kF64TbC;
@@ -702,7 +709,7 @@ k_fused_f64(int npair, int64_t lena, int64_t lenb,
   const __amdgpu_buffer_rsrc_t rpb = buffer_of(ptb, static_cast<int64_t>(K) * lenb * 4);
   const __amdgpu_buffer_rsrc_t rfb = buffer_of(fwb, static_cast<int64_t>(K) * lenb * 4);
 
-  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x) {
+  {
     load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
                             kFThreads);
     for (int64_t x = tid; x < 2 * lenb; x += kFThreads) accrow[x] = 0.0;
@@ -795,6 +802,23 @@ k_fused_f64(int npair, int64_t lena, int64_t lenb,
     for (int64_t x = tid; x < 2 * lenb; x += kFThreads) dst[x] = accrow[x];
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(kFThreads, 2)
+k_fused_f64(int npair, int64_t lena, int64_t lenb,
+            const double* __restrict__ amat, const int32_t* __restrict__ pta,
+            const int32_t* __restrict__ ptb, const int32_t* __restrict__ fwa,
+            const int32_t* __restrict__ fwb, const int32_t* __restrict__ sla,
+            int nslot, const double* __restrict__ coeff,
+            double* __restrict__ slots, double* __restrict__ rowacc,
+            const int32_t* __restrict__ done) {
+  extern __shared__ __align__(16) char smem[];
+  if (done && *done) return;
+  double areg[kF64KSteps];
+  f64_a_regs(npair, amat, areg);
+  for (int64_t ia = blockIdx.x; ia < lena; ia += gridDim.x)
+    f64_row(ia, npair, lena, lenb, pta, ptb, fwa, fwb, sla, nslot, coeff,
+            slots, rowacc, smem, areg);
 }
 
 // ---------------- FP64 split kernel over the live pair rows ------------------
@@ -1300,14 +1324,53 @@ k_fused_group(const GroupSector* __restrict__ gs, int nsec, int64_t rows,
     fused_row<NW, false>(gr - sc.row0, sc.norb, sc.compressed, sc.npair,
                          sc.lena, sc.lenb, sc.amat, sc.pta, sc.ptb, sc.fwa,
                          sc.fwb, sc.sla, sc.nslot,
-                         step == 0 ? sc.coeff0 : sc.term, sc.slots, sc.rowacc,
-                         smem, ahi, alo);
+                         static_cast<const float*>(step == 0 ? sc.coeff0 : sc.term),
+                         static_cast<float*>(sc.slots),
+                         static_cast<float*>(sc.rowacc), smem, ahi, alo);
+  }
+}
+
+// The complex128 counterpart: the row body of k_fused_f64 over the rows of
+// every sector of the group, A' registers reloaded where a workgroup's run
+// of rows crosses into another sector.
+__global__ void __launch_bounds__(kFThreads, 2)
+k_fused_group_f64(const GroupSector* __restrict__ gs, int nsec, int64_t rows,
+                  int step) {
+  extern __shared__ __align__(16) char smem[];
+  double areg[kF64KSteps];
+  const int64_t per = (rows + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * per;
+  const int64_t r1 = r0 + per < rows ? r0 + per : rows;
+  int lo = 0;
+  if (r0 < rows) {
+    int hi = nsec - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (gs[mid].row0 <= r0) lo = mid;
+      else hi = mid - 1;
+    }
+  }
+  int cur = -1;
+  for (int64_t gr = r0; gr < r1; ++gr) {
+    while (lo + 1 < nsec && gs[lo + 1].row0 <= gr) ++lo;
+    const GroupSector& sc = gs[lo];
+    if (step == 0 && sc.apply_out == nullptr) continue;
+    if (lo != cur) {
+      f64_a_regs(sc.npair, sc.amat, areg);
+      cur = lo;
+    }
+    f64_row(gr - sc.row0, sc.npair, sc.lena, sc.lenb, sc.pta, sc.ptb, sc.fwa,
+            sc.fwb, sc.sla, sc.nslot,
+            static_cast<const double*>(step == 0 ? sc.coeff0 : sc.term),
+            static_cast<double*>(sc.slots), static_cast<double*>(sc.rowacc),
+            smem, areg);
   }
 }
 
 // Step 0: apply_out = rowacc + slots.  Step k >= 1: term = (ar + i ai)
 // (rowacc + slots) in place and result += term in FP64.  Block gb belongs
 // to the last sector with blk0 <= gb.
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_slot_reduce_group(const GroupSector* __restrict__ gs, int nsec, int step,
                     double ar, double ai) {
@@ -1319,16 +1382,17 @@ k_slot_reduce_group(const GroupSector* __restrict__ gs, int nsec, int step,
     else hi = mid - 1;
   }
   const GroupSector& sc = gs[lo];
-  float* out = step == 0 ? sc.apply_out : sc.term;
+  T* out = static_cast<T*>(step == 0 ? sc.apply_out : sc.term);
   if (out == nullptr) return;
   const int64_t idx = (gb - sc.blk0) * kThreads + threadIdx.x;
   if (idx >= sc.lena * sc.lenb) return;
   const int64_t ia = idx / sc.lenb;
   const int64_t ib = idx - ia * sc.lenb;
-  (void)slot_element<float>(idx, ia, ib, sc.nslot, sc.lenb, sc.slots,
-                            sc.rowacc, step == 0 ? 1.0 : ar,
-                            step == 0 ? 0.0 : ai, 0.0, 0.0, nullptr, out,
-                            step == 0 ? nullptr : sc.result);
+  (void)slot_element<T>(idx, ia, ib, sc.nslot, sc.lenb,
+                        static_cast<const T*>(sc.slots),
+                        static_cast<const T*>(sc.rowacc), step == 0 ? 1.0 : ar,
+                        step == 0 ? 0.0 : ai, 0.0, 0.0, nullptr, out,
+                        step == 0 ? nullptr : sc.result);
 }
 
 size_t row_tables_bytes() {
@@ -1518,11 +1582,25 @@ int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
 }
 
 // Grouped launches (batch code): rows of every sector of the group, then
-// their reductions; rows4 row body (two 4-wave workgroups per CU measured
-// faster here than one 8-wave one), LDS sized for the longest beta row.
-int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
-                      int64_t max_lenb, int step, hipStream_t st) {
+// their reductions; complex64: rows4 row body (two 4-wave workgroups per CU
+// measured faster here than one 8-wave one); complex128: the k_fused_f64
+// row body; LDS sized for the longest beta row.
+int launch_group_rows(int precision, const GroupSector* d_secs, int nsec,
+                      int64_t rows, int64_t max_lenb, int step,
+                      hipStream_t st) {
   if (rows == 0 || nsec == 0) return FQEHIP_OK;
+  const unsigned grid = static_cast<unsigned>(rows < 4096 ? rows : 4096);
+  if (precision == FQEHIP_C128) {
+    const size_t lds = f64_lds_bytes(max_lenb);
+    if (lds > kLdsLimit) {
+      set_error("grouped rows: lenb=%lld exceeds the LDS row accumulator",
+                static_cast<long long>(max_lenb));
+      return FQEHIP_ERR_INVALID;
+    }
+    k_fused_group_f64<<<grid, kFThreads, lds, st>>>(d_secs, nsec, rows, step);
+    FQEHIP_CHECK_LAUNCH("k_fused_group_f64");
+    return FQEHIP_OK;
+  }
   const size_t lds = bf16_lds_bytes(max_lenb);
   if (lds > kLdsLimit) {
     set_error("grouped rows: lenb=%lld exceeds the LDS row accumulator",
@@ -1531,21 +1609,25 @@ int launch_group_rows(const GroupSector* d_secs, int nsec, int64_t rows,
   }
   // about four workgroups per CU at a time, several rounds: runs of
   // consecutive rows while the load stays balanced
-  const unsigned grid = static_cast<unsigned>(rows < 4096 ? rows : 4096);
   k_fused_group<4, 2><<<grid, 256, lds, st>>>(d_secs, nsec, rows, step);
   FQEHIP_CHECK_LAUNCH("k_fused_group");
   return FQEHIP_OK;
 }
 
-int launch_group_reduce(const GroupSector* d_secs, int nsec, int64_t blocks,
-                        int step, double ar, double ai, hipStream_t st) {
+int launch_group_reduce(int precision, const GroupSector* d_secs, int nsec,
+                        int64_t blocks, int step, double ar, double ai,
+                        hipStream_t st) {
   if (blocks == 0 || nsec == 0) return FQEHIP_OK;
   if (blocks > 0x7fffffffLL) {
     set_error("grouped reduction: %lld blocks", static_cast<long long>(blocks));
     return FQEHIP_ERR_INVALID;
   }
-  k_slot_reduce_group<<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(
-      d_secs, nsec, step, ar, ai);
+  if (precision == FQEHIP_C128)
+    k_slot_reduce_group<double><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(
+        d_secs, nsec, step, ar, ai);
+  else
+    k_slot_reduce_group<float><<<static_cast<unsigned>(blocks), kThreads, 0, st>>>(
+        d_secs, nsec, step, ar, ai);
   FQEHIP_CHECK_LAUNCH("k_slot_reduce_group");
   return FQEHIP_OK;
 }

@@ -152,9 +152,10 @@ def autotune(specs: Sequence[SectorSpec], dtype: torch.dtype = torch.complex64,
 
 class BatchedTaylor:
     """One apply + ``nterms``-term Taylor series of exp(-i t H) for every
-    sector, recorded into one HIP graph.  With ``group`` (complex64), the
-    sectors of the rows4/rows8 variants run as one group: one row launch
-    and one reduction per step for all of them."""
+    sector, recorded into one HIP graph.  With ``group``, the sectors of
+    the fused row variants without the split (complex64 rows4/rows8,
+    complex128 FP64 rows) run as one group: one row launch and one
+    reduction per step for all of them."""
 
     def __init__(self, specs: Sequence[SectorSpec], time_step: float,
                  nterms: int = 10, dtype: torch.dtype = torch.complex64,

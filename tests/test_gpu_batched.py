@@ -119,6 +119,49 @@ def test_grouped_batch_matches_oracle_and_sequential(fqe):
     bt.close()
 
 
+@pytest.mark.parametrize("contraction", ["fp64", "ozaki"])
+def test_grouped_batch_complex128(fqe, contraction):
+    """complex128 sectors of the rows variant (the FP64 row kernel) run as
+    one group beside split (FP64 or Ozaki-digit) and unfused sectors on
+    their own kernels: every sector against the oracle to 1e-11 (Ozaki
+    split sectors: 1e-9), replays against each other, and the group
+    against the per-sector kernels of run_sequential."""
+    import torch
+    from fqe_amd import batched
+    specs = batched.random_specs(24, seed=46, norb_range=(4, 9))
+    cycle = ["rows4", "split", "rows4", "unfused"]
+    tiles, want = {}, 0
+    for i, sp in enumerate(specs):
+        key = f"{sp.norb},{sp.nalpha},{sp.nbeta}"
+        if key not in tiles:
+            tiles[key] = cycle[i % len(cycle)]
+    print("variants", tiles, flush=True)
+    bt = batched.BatchedTaylor(specs, 0.1, nterms=10, dtype=torch.complex128,
+                               tiles=tiles, nstreams=4, group=True,
+                               contraction=contraction)
+    bt.build()
+    want = sum(1 for p in bt.sectors if p.variant == "rows4")
+    assert bt.grouped() == want > 0
+    refs = [_oracle(sp, 0.1, 10) for sp in specs]
+    tol = 1e-9 if contraction == "ozaki" else 1e-11
+    for rep in range(2):
+        bt.launch()
+        got = bt.results()
+        for (ap, ev), (rap, rev), sp in zip(got, refs, specs):
+            assert rel(ap, rap) < tol, (rep, sp.norb, sp.nalpha, sp.nbeta)
+            assert rel(ev, rev) < tol, (rep, sp.norb, sp.nalpha, sp.nbeta)
+        if rep == 0:
+            first = got
+        else:
+            for (a0, e0), (a1, e1) in zip(first, got):
+                assert rel(a1, a0) < 1e-13 and rel(e1, e0) < 1e-13
+        torch.zeros(1000, device="cuda").sum().item()
+    bt.run_sequential()
+    for (a0, e0), (a1, e1) in zip(first, bt.results()):
+        assert rel(a1, a0) < 1e-13 and rel(e1, e0) < 1e-13
+    bt.close()
+
+
 def test_grouped_batch_without_apply(fqe):
     """Items with d_apply_out NULL (Taylor series only) inside a group: the
     grouped step-0 launches skip them, their Taylor sums still match the
