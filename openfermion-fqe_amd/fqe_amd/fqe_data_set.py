@@ -38,6 +38,22 @@ def _set_keys(nele: int, norb: int):
             if nele - nb <= norb and nb <= norb]
 
 
+def _check_fits(nbytes: int, what: str) -> None:
+    """MemoryError, before any allocation, when a workspace of the
+    spin-orbital path exceeds the device memory that is free (torch's
+    cached blocks included): the path keeps the (2 norb)^2 pair planes of
+    every sector in HBM at once and has no chunking over pairs."""
+    dev = _lib.device()
+    free, _ = torch.cuda.mem_get_info(dev)
+    free += torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
+    if nbytes > free:
+        raise MemoryError(
+            f"{what} needs {nbytes / 2**30:.1f} GiB of device memory and "
+            f"{free / 2**30:.1f} GiB are free: the spin-orbital path holds "
+            "all (2 norb)^2 pair planes of every sector at once (no "
+            "chunking over pairs)")
+
+
 class FqeDataSet:
     """reference ``src/fqe/fqe_data_set.py:35-60``"""
 
@@ -98,6 +114,8 @@ class FqeDataSet:
     def _dvec(self, vec: torch.Tensor) -> torch.Tensor:
         """[(2 norb)^2, L] complex128 D[pq] = E_pq C."""
         m = 2 * self._norb
+        # planar planes, then their complex128 copy
+        _check_fits(2 * m * m * vec.numel() * 16, "the spin-orbital dvec")
         d = torch.empty((m * m, 2, vec.numel()), dtype=torch.float64,
                         device=vec.device)
         _lib.call("fqehip_set_dvec", self._norb, self._nele, _lib.ptr(vec),
@@ -140,6 +158,7 @@ class FqeDataSet:
                 numpy.ascontiguousarray(v.reshape(m * m, m * m))).to(dev)
             nbytes = int(_lib.call("fqehip_set_workspace_bytes", self._norb,
                                    self._nele))
+            _check_fits(nbytes, "the spin-orbital two-body apply")
             ws = torch.empty(max(nbytes, 16), dtype=torch.uint8, device=dev)
             _lib.call("fqehip_set_apply", self._norb, self._nele,
                       _lib.ptr(th1), _lib.ptr(tv), _lib.ptr(vec),

@@ -192,6 +192,22 @@ def test_set_errors(fqe):
                     numpy.zeros((m,) * 6)))
 
 
+def test_set_too_large_raises_memory_error(fqe):
+    """norb = nele = 14: 40 M determinants over 15 sectors, so the two-body
+    apply's (2 norb)^2-pair workspace (about 1 TB) and the RDM dvec exceed
+    HBM; both raise MemoryError before allocating anything of that size."""
+    print_case("too large", nele=14, norb=14)
+    norb = nele = 14
+    data = {(nele - nb, nb): fqe.FqeData(nele - nb, nb, norb)
+            for nb in range(nele + 1)}
+    wset = fqe.FqeDataSet(nele, norb, data)
+    m = 2 * norb
+    with pytest.raises(MemoryError, match="no chunking over pairs"):
+        wset.apply((numpy.zeros((m, m)), numpy.zeros((m,) * 4)))
+    with pytest.raises(MemoryError, match="spin-orbital dvec"):
+        wset.rdm1()
+
+
 # --------------------------------------------------------------------------
 # Wavefunction-level routing: General / GSOHamiltonian on spin-broken
 # wavefunctions go through FqeDataSet per particle number (reference
