@@ -566,6 +566,42 @@ def roofline_entry(name, per_ms, work, mode, traffic=None):
             "bytes_per_launch": w["bytes"]}
 
 
+def api_rates(coeff, h1, h2, norb, na, nb, reps=10):
+    """Applies/s of the drop-in Python call ``Wavefunction.apply`` with the
+    coefficients resident in HBM (a new Wavefunction per call, as in the
+    reference; the operator's device form cached on the Hamiltonian),
+    Ozaki-digit and FP64 contractions."""
+    import numpy
+    import torch
+    import fqe_amd as fqe
+    from fqe_amd import settings
+    key = (na + nb, na - nb)
+    saved = settings.contraction
+    out = {}
+    try:
+        ham = fqe.get_restricted_hamiltonian((h1, h2))
+        wfn = fqe.Wavefunction([[na + nb, na - nb, norb]],
+                               dtype=numpy.complex128)
+        wfn.set_wfn(strategy="from_data",
+                    raw_data={key: torch.from_numpy(coeff)})
+        for mode in ("ozaki", "fp64"):
+            settings.contraction = mode
+            res = wfn.apply(ham)                  # warm-up
+            del res
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                res = wfn.apply(ham)
+                del res
+            torch.cuda.synchronize()
+            out[mode] = round(reps / (time.perf_counter() - t0), 2)
+    finally:
+        settings.contraction = saved
+    del wfn, ham
+    torch.cuda.empty_cache()
+    return out
+
+
 def host_boundary_rate(coeff, h1, h2, norb, na, nb, reps=3):
     """Headline-engine applies/s when the caller hands over and takes back
     host arrays (Wavefunction.set_wfn from numpy, apply, get_coeff): the
@@ -1056,6 +1092,7 @@ def main():
     if not args.no_cfg1 and (norb, na, nb) == (14, 7, 7):
         cfg1 = cfg1_line(not args.no_cpu_baseline)
     host_rate = host_boundary_rate(coeff, h1, h2, norb, na, nb)
+    api = api_rates(coeff, h1, h2, norb, na, nb)
     evolve = None
     if not args.no_evolve and (norb, na, nb) == (14, 7, 7):
         evolve = time_evolve_lines(not args.no_cpu_baseline)
@@ -1116,6 +1153,7 @@ def main():
             "roofline": roofline_entry(dom_m, bf["per"], work_m, "mixed",
                                        traffic.get(dom_m))},
         "host_boundary_applies_per_s": round(host_rate, 3),
+        "wavefunction_apply_applies_per_s": api,
         "cpu_baseline": cpu,
     }
     if ref_out is not None:
