@@ -11,6 +11,8 @@ import json
 import math
 import os
 
+import itertools
+
 import numpy
 import pytest
 import scipy.linalg
@@ -349,6 +351,64 @@ def test_rdm12_oracle_matches_fqe_golden_and_energy(golden):
     h1 = rng.normal(size=(4, 4))
     e = numpy.vdot(c, fo.apply_12(c, h1, h2, g))
     assert abs(numpy.sum(h1 * r1) + numpy.einsum("ijkl,ijkl->", h2, r2) - e) < 1e-12
+
+
+def _chemist_dense(h1, eri, norb, na, nb):
+    """Dense H of the textbook chemist-notation Hamiltonian
+    sum_pq h1[p,q] sum_s a+_ps a_qs
+      + 1/2 sum_pqrs (pq|rs) sum_{s,t} a+_ps a+_rt a_st a_qs
+    on FQE's (na, nb) sector basis, built operator string by operator string
+    (independent of FQE's h2 convention)."""
+    astr = jw_dense._strings(norb, na)
+    bstr = jw_dense._strings(norb, nb)
+    lenb = len(bstr)
+    basis = {sa | (sb << norb): ia * lenb + ib
+             for ia, sa in enumerate(astr) for ib, sb in enumerate(bstr)}
+    mat = numpy.zeros((len(basis), len(basis)), dtype=numpy.complex128)
+    terms = []
+    for p_ in range(norb):
+        for q in range(norb):
+            for s_ in range(2):
+                terms.append((h1[p_, q], ((p_ + s_ * norb, True),
+                                          (q + s_ * norb, False))))
+    for p_, q, r, t in itertools.product(range(norb), repeat=4):
+        for s_ in range(2):
+            for u in range(2):
+                terms.append((0.5 * eri[p_, q, r, t],
+                              ((p_ + s_ * norb, True), (r + u * norb, True),
+                               (t + u * norb, False), (q + s_ * norb, False))))
+    for state, col in basis.items():
+        for coef, ops in terms:
+            res = jw_dense._apply_string(state, ops)
+            if res is not None:
+                mat[basis[res[0]], col] += res[1] * coef
+    return mat
+
+
+@pytest.mark.parametrize("norb,na,nb,seed", [(3, 2, 1, 7), (4, 2, 1, 8),
+                                             (4, 2, 2, 9)])
+def test_physics_to_fqe_h2_is_the_chemist_hamiltonian(norb, na, nb, seed):
+    """The conversion of the synthetic ERIs (ij|kl) to FQE's restricted h2
+    (physics_to_fqe_h2, after the reference's integrals_to_fqe_restricted)
+    gives the Hamiltonian of the textbook chemist-notation formula, built
+    here from its own operator strings; a transposed index, a dropped 1/2
+    or a flipped sign in the conversion fails this."""
+    rng = numpy.random.default_rng(seed)
+    h1, _ = fo.synthetic_hamiltonian(norb, 2, rng)
+    # ERIs with the 8-fold symmetry of real orbitals, as in the workload
+    eri = numpy.zeros((norb,) * 4)
+    for _ in range(3):
+        lk = rng.normal(0.0, 0.1, size=(norb, norb))
+        lk = lk + lk.T
+        eri += numpy.einsum("ij,kl->ijkl", lk, lk)
+    ref = _chemist_dense(h1, eri, norb, na, nb)
+    h2 = fo.physics_to_fqe_h2(eri)
+    dense = jw_dense.dense_hamiltonian(h1, h2, norb, na, nb)
+    assert numpy.max(numpy.abs(dense - ref)) < 1e-13
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    got = fo.apply_12(c, h1, h2, g).reshape(-1)
+    assert numpy.linalg.norm(got - ref @ c.reshape(-1)) < 1e-12 * numpy.linalg.norm(got)
 
 
 @pytest.mark.parametrize("norb,na,nb,seed,k", [(8, 4, 4, 0, 8), (6, 2, 3, 5, 3)])
