@@ -61,7 +61,8 @@ def _apply(fqe, c, h1, h2, na, nb, norb, mode, op=None):
 
 @pytest.mark.parametrize("na,nb,norb", [(1, 1, 3), (2, 2, 4), (4, 4, 8),
                                         (3, 5, 9), (5, 5, 10), (7, 3, 10),
-                                        (2, 6, 8), (6, 6, 12), (5, 6, 11)])
+                                        (2, 6, 8), (6, 6, 12), (5, 6, 11),
+                                        (6, 7, 13)])
 def test_ozaki_apply_matches_fp64_and_oracle(fqe, na, nb, norb):
     import ctypes
     from fqe_amd import _lib, engine
@@ -136,3 +137,36 @@ def test_ozaki_norb14_vs_fp64_and_reference_c(fqe):
     e1, e2 = rel(oz, fp), rel(oz, ref)
     print(f"  ozaki vs fp64 {e1:.3e}, vs reference C {e2:.3e}", flush=True)
     assert e1 < GATE and e2 < GATE
+
+
+def test_ozaki_norb15_global_beta_rows(fqe):
+    """norb=15 7a8b (41.4 M determinants), the largest sector the digit
+    split kernel takes (121 of its 128 pair rows): row ia (103 KB) no longer
+    fits in LDS beside the operator digits, so the beta gather reads it from
+    global memory; against the FP64 engine (here its unfused path with the
+    LDS-resident pair GEMM, 8 row tiles) at the 1e-9 gate."""
+    import ctypes
+    import torch
+    from fqe_amd import _lib, engine
+    norb, na, nb = 15, 7, 8
+    print("norb=15 ozaki vs fp64", flush=True)
+    rng = numpy.random.default_rng(1507)
+    h1, h2 = fo.synthetic_hamiltonian(norb, norb, rng)
+    g = fo.OracleGraph(na, nb, norb)
+    c = fo.synthetic_coeff(g.lena, g.lenb, rng)
+    with _contraction("ozaki"):
+        d = fqe.FqeData(na, nb, norb, dtype=numpy.complex128)
+        op = engine.prepare_operator((h1, h2), norb, _lib.device())
+        s, keep = engine.sector_struct(d.get_fcigraph(), op, _lib.C128)
+        assert s.tile == _lib.TILE_OZAKI and s.hbb_f64
+        assert _lib.lib().fqehip_fused_supported(
+            _lib.C128, ctypes.byref(s), ctypes.byref(op.c_struct()))
+        del keep, d
+    oz = _apply(fqe, c, h1, h2, na, nb, norb, "ozaki")
+    torch.cuda.empty_cache()
+    fp = _apply(fqe, c, h1, h2, na, nb, norb, "fp64")
+    torch.cuda.empty_cache()
+    err = rel(oz, fp)
+    print(f"  ozaki vs fp64 {err:.3e}", flush=True)
+    assert err < GATE
+
