@@ -201,7 +201,9 @@ typedef struct fqehip_sector {
  * live rows of each alpha string (the default, tile 0) */
 #define FQEHIP_TILE_SPLIT_ALLROWS 16
 /* complex128 split path (hbb_f64 set): the row kernel with a shared,
- * double-buffered D tile instead of column-owned waves (the default) */
+ * double-buffered D tile instead of column-owned waves (the default).
+ * Gather depth npair + 1: up to 128 for the column-owned and the digit
+ * kernels (norb <= 15 compressed), up to 112 with this tile. */
 #define FQEHIP_TILE_SPLIT_F64_DTILE 32
 /* complex128 split path (hbb_f64 set): the alpha-row contraction as exact
  * products of 8-bit digits on the int8 matrix cores (Ozaki scheme: A' in

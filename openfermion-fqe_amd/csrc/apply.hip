@@ -602,7 +602,7 @@ int launch_gemm_f64(int m, int k, const double* a, int64_t ncol,
 // Defined in fused.hip.
 bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb);
 bool split_supported(int npair, int64_t lena, int64_t lenb);
-bool split_f64_supported(int npair, int64_t lena, int64_t lenb);
+bool split_f64_supported(int npair, int64_t lena, int64_t lenb, int tile);
 int launch_fused(int precision, int norb, int npair, int64_t lena, int64_t lenb,
                  const double* a, const int32_t* pta, const int32_t* ptb,
                  const int32_t* fwa, const int32_t* fwb, const int32_t* sla,
@@ -863,7 +863,7 @@ extern "C" int fqehip_fused_supported(int precision, const fqehip_sector* s,
            (s->hbb_oz ? beta_oz_shape_ok(s->lena, s->lenb)
                       : beta_gemm_f64_shape_ok(s->lena, s->lenb));
   if (precision == FQEHIP_C128 && s->hbb_f64)
-    return split_f64_supported(s->npair, s->lena, s->lenb) &&
+    return split_f64_supported(s->npair, s->lena, s->lenb, s->tile) &&
            beta_gemm_f64_shape_ok(s->lena, s->lenb);
   return fused_supported(precision, s->npair, s->lena, s->lenb);
 }

@@ -1023,10 +1023,15 @@ k_split_live_f64(int norb, int compressed, int npair, int64_t lena,
 // gather overlaps another wave's MFMAs.
 constexpr int kCF64Threads = 256;
 constexpr int kCF64Cols = 64;               // columns per chunk (4 waves x 16)
+constexpr int kCF64KMax = 128;              // npair + 1 limit (norb 15)
 constexpr int kCF64AS = kLiveRows + 4;      // doubles per LDS A' k-row (padding)
 constexpr int kCF64Batch = 7;               // k steps gathered per load batch
 constexpr int kCF64Parts = 4;               // k-step parts held in registers
 
+// KMAX: the gather depth the kernel is built for, kF64KMax (112: the
+// sectors up to norb 14) or kCF64KMax (128: norb 15); the narrower build
+// keeps fewer k steps in registers.
+template <int KMAX>
 __global__ void __launch_bounds__(kCF64Threads, 2)
 k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
                  int64_t lenb, const double* __restrict__ amat,
@@ -1035,7 +1040,7 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
                  int nslot, const double* __restrict__ coeff,
                  double* __restrict__ slots, double* __restrict__ rowacc,
                  const int32_t* __restrict__ done) {
-  static_assert(kF64KSteps % kCF64Parts == 0, "parts of the k steps");
+  static_assert((KMAX / 4) % kCF64Parts == 0, "parts of the k steps");
   extern __shared__ __align__(16) char smem[];
   if (done && *done) return;
   const int tid = threadIdx.x;
@@ -1044,8 +1049,8 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
   const int K = npair;
   const int Kg = K + 1;                     // gathered rows: pairs + C
   const int ksteps = (Kg + 3) / 4;
-  double* alds = reinterpret_cast<double*>(smem);     // [kF64KMax][kCF64AS]
-  double* drow = alds + kF64KMax * kCF64AS;
+  double* alds = reinterpret_cast<double*>(smem);     // [KMAX][kCF64AS]
+  double* drow = alds + KMAX * kCF64AS;
   RowTables rt;
   rt.sbase = reinterpret_cast<int64_t*>(drow + kF64KPad);
   rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
@@ -1087,7 +1092,7 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
     }
     __syncthreads();
     // A'[live row m][k] into LDS as [k][m]
-    for (int x = tid; x < kF64KMax * kLiveRows; x += kCF64Threads) {
+    for (int x = tid; x < KMAX * kLiveRows; x += kCF64Threads) {
       const int k = x / kLiveRows;
       const int m = x - k * kLiveRows;
       const int p = lpair[m];
@@ -1118,7 +1123,7 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
       }
 #pragma unroll 1
       for (int hk = 0; hk < kCF64Parts; ++hk) {
-        constexpr int kH = kF64KSteps / kCF64Parts;
+        constexpr int kH = (KMAX / 4) / kCF64Parts;
         if (hk * kH >= ksteps) break;      // parts no MFMA reads
         double bre[kH], bim[kH];
 #pragma unroll
@@ -1415,8 +1420,8 @@ size_t split_live_f64_lds_bytes() {
          kLiveRows * sizeof(int32_t);
 }
 
-size_t split_cols_f64_lds_bytes() {
-  return (kF64KMax * kCF64AS + kF64KPad) * sizeof(double) + row_tables_bytes() +
+size_t split_cols_f64_lds_bytes(int kmax = kCF64KMax) {
+  return (kmax * kCF64AS + kF64KPad) * sizeof(double) + row_tables_bytes() +
          kLiveRows * sizeof(int32_t);
 }
 
@@ -1448,11 +1453,15 @@ bool fused_supported(int precision, int npair, int64_t lena, int64_t lenb) {
 
 // FP64 split: the live-rows kernel's LDS does not grow with lenb (the row
 // accumulator is the global rowacc), and its gather depth is kF64KMax.
-bool split_f64_supported(int npair, int64_t lena, int64_t lenb) {
+// The FP64 split kernels: the column-owned k_split_cols_f64 (default tile)
+// gathers up to kCF64KMax pair rows, the D-tile k_split_live_f64 kF64KMax.
+bool split_f64_supported(int npair, int64_t lena, int64_t lenb, int tile) {
   const int64_t lim = int64_t(1) << 31;
-  return npair + 1 <= kF64KMax && static_cast<int64_t>(npair) * lenb * 4 < lim &&
-         lena * lenb * 2 * 8 < lim &&
-         split_live_f64_lds_bytes() <= kLdsLimit;
+  const bool dtile = tile == FQEHIP_TILE_SPLIT_F64_DTILE;
+  const int kmax = dtile ? kF64KMax : kCF64KMax;
+  const size_t lds = dtile ? split_live_f64_lds_bytes() : split_cols_f64_lds_bytes();
+  return npair + 1 <= kmax && static_cast<int64_t>(npair) * lenb * 4 < lim &&
+         lena * lenb * 2 * 8 < lim && lds <= kLdsLimit;
 }
 
 bool split_supported(int npair, int64_t lena, int64_t lenb) {
@@ -1469,10 +1478,11 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
   if (split && precision == FQEHIP_C128) {
-    if (!split_f64_supported(npair, lena, lenb) ||
+    if (!split_f64_supported(npair, lena, lenb, tile) ||
         norb * norb / 4 + 1 > kLiveRows) {
-      set_error("the FP64 split apply needs npair + 1 <= %d and norb^2/4 + 1 <= %d",
-                kF64KMax, kLiveRows);
+      set_error("the FP64 split apply needs npair + 1 <= %d (%d with the "
+                "shared D tile) and norb^2/4 + 1 <= %d",
+                kCF64KMax, kF64KMax, kLiveRows);
       return FQEHIP_ERR_INVALID;
     }
     if (tile == FQEHIP_TILE_SPLIT_F64_DTILE) {
@@ -1483,10 +1493,18 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
       FQEHIP_CHECK_LAUNCH("k_split_live_f64");
       return FQEHIP_OK;
     }
-    k_split_cols_f64<<<fused_grid(lena), kCF64Threads, split_cols_f64_lds_bytes(), st>>>(
-        norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
-        static_cast<const double*>(coeff), static_cast<double*>(slots),
-        static_cast<double*>(rowacc), done);
+    if (npair + 1 <= kF64KMax)
+      k_split_cols_f64<kF64KMax><<<fused_grid(lena), kCF64Threads,
+                                   split_cols_f64_lds_bytes(kF64KMax), st>>>(
+          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+          static_cast<const double*>(coeff), static_cast<double*>(slots),
+          static_cast<double*>(rowacc), done);
+    else
+      k_split_cols_f64<kCF64KMax><<<fused_grid(lena), kCF64Threads,
+                                    split_cols_f64_lds_bytes(kCF64KMax), st>>>(
+          norb, compressed, npair, lena, lenb, a, pta, ptb, fwa, sla, nslot,
+          static_cast<const double*>(coeff), static_cast<double*>(slots),
+          static_cast<double*>(rowacc), done);
     FQEHIP_CHECK_LAUNCH("k_split_cols_f64");
     return FQEHIP_OK;
   }

@@ -21,6 +21,7 @@ LIBPATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
 C128 = 0
 C64 = 1
 
+TILE_SPLIT_F64_DTILE = 32   # FQEHIP_TILE_SPLIT_F64_DTILE
 TILE_OZAKI = 64   # FQEHIP_TILE_OZAKI
 
 OK = 0

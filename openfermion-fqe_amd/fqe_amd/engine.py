@@ -255,9 +255,10 @@ def sector_struct(graph, op: PreparedOperator,
     hbb = hbb64 = None
     if fwa is not None and settings.split and precision == _lib.C64:
         hbb = op.beta_block(graph)
-    # FP64 split kernel limits: gather depth npair + 1 <= 112 (128 for the
-    # digit kernel), at most 64 live pair rows (norb^2/4 + 1)
-    kmax = 128 if settings.contraction == "ozaki" else 112
+    # split kernel limits: gather depth npair + 1 <= 128 (112 for the FP64
+    # shared-D-tile variant), at most 64 live pair rows (norb^2/4 + 1)
+    kmax = 112 if (settings.contraction == "fp64" and
+                   settings.split_tile_fp64 == _lib.TILE_SPLIT_F64_DTILE) else 128
     if (fwa is not None and settings.split_fp64 and precision == _lib.C128
             and op.npair + 1 <= kmax and graph.norb() ** 2 // 4 + 1 <= 64):
         blk = op.beta_block(graph)

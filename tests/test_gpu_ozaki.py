@@ -143,8 +143,10 @@ def test_ozaki_norb15_global_beta_rows(fqe):
     """norb=15 7a8b (41.4 M determinants), the largest sector the digit
     split kernel takes (121 of its 128 pair rows): row ia (103 KB) no longer
     fits in LDS beside the operator digits, so the beta gather reads it from
-    global memory; against the FP64 engine (here its unfused path with the
-    LDS-resident pair GEMM, 8 row tiles) at the 1e-9 gate."""
+    global memory.  Against the FP64 engine (the column-owned FP64 split
+    kernel at its widest gather, 121 of 128 rows) at the 1e-9 gate, and the
+    FP64 split result against the FP64 unfused path (its LDS-resident pair
+    GEMM, 8 row tiles) to 1e-12."""
     import ctypes
     import torch
     from fqe_amd import _lib, engine
@@ -169,4 +171,15 @@ def test_ozaki_norb15_global_beta_rows(fqe):
     err = rel(oz, fp)
     print(f"  ozaki vs fp64 {err:.3e}", flush=True)
     assert err < GATE
+    from fqe_amd import settings
+    saved = settings.split_fp64
+    settings.split_fp64 = False
+    try:
+        unf = _apply(fqe, c, h1, h2, na, nb, norb, "fp64")
+    finally:
+        settings.split_fp64 = saved
+    torch.cuda.empty_cache()
+    err2 = rel(fp, unf)
+    print(f"  fp64 split vs fp64 unfused {err2:.3e}", flush=True)
+    assert err2 < 1e-12
 
