@@ -645,9 +645,6 @@ namespace {
 constexpr int kBgTM = 128;                  // tile rows (M)
 constexpr int kBgTN = 64;                   // tile columns (N)
 constexpr int kBgK = 64;                    // K step
-constexpr int kBgS = kBgK + 16;             // bytes per LDS row (padding)
-constexpr int kBgC = 6 * kBgTM * 4;         // 16-byte C staging loads per K step
-constexpr int kBgH = 5 * kBgTN * 4;         // 16-byte Hbb staging loads per K step
 constexpr int kBgXcds = 8;                  // XCDs of an MI355X
 constexpr int kBgGroup = 8;                 // row tiles per group of the order
 
@@ -769,22 +766,42 @@ k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
 // GEMM tile: 128 M rows (64 alpha strings, real and imaginary rows) x 64
 // columns per workgroup of 4 waves (2 x 2, 64 x 32 per wave: 4 x 2 MFMA
 // tiles of 16 x 16, six accumulators each: 192 accumulator registers, so
-// one wave per SIMD with the accumulators in AGPRs), K in steps of 64
-// staged through LDS (87 KB, one workgroup per CU) with the next step's
-// global loads in flight.  Against 32 x 32 wave tiles at two waves per SIMD
-// this halves the LDS operand reads per MFMA (each A fragment serves two
-// column tiles, each B fragment four row tiles) and the barriers per MFMA.
+// one wave per SIMD with the accumulators in AGPRs), K in steps of 64.
+// Against 32 x 32 wave tiles at two waves per SIMD this halves the LDS
+// operand reads per MFMA (each A fragment serves two column tiles, each B
+// fragment four row tiles) and the barriers per MFMA.
+// The operand tiles are double-buffered in LDS (2 x 68 KB, one workgroup
+// per CU) and filled by direct buffer-to-LDS loads (buffer_load_dwordx4
+// ... lds: lane L of an instruction writes bytes 16 L .. 16 L + 15 past the
+// instruction's LDS base), so no staging registers: one instruction moves
+// 16 rows x 64 bytes of one digit plane.  Rows are 64 bytes, unpadded; the
+// four 16-byte chunks of row r sit XOR-swizzled by (r / 4) % 4, so the 16
+// lanes of a fragment read (rows r..r+15 at one chunk) hit distinct banks.
+// Step ks + 1's loads go out after the barrier of step ks and land in the
+// other buffer while step ks's MFMAs run; one wait and one barrier per
+// step.
+constexpr int kBdBuf = (6 * kBgTM + 5 * kBgTN) * kBgK;   // bytes per buffer
+constexpr int kBdNoRow = 0x7ff00000;        // offset of a row past the matrix
+
+__device__ __forceinline__ int bd_off(int row, int chunk) {
+  return row * kBgK + 16 * (chunk ^ ((row >> 2) & 3));
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
 k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
-               const uint8_t* __restrict__ cdig,
-               const double* __restrict__ cscale,
-               const uint8_t* __restrict__ hdig,
-               const double* __restrict__ hscale,
-               const double* __restrict__ hcorr, double* __restrict__ out,
-               const int32_t* __restrict__ done, int tm, int tn, int per_xcd) {
-  extern __shared__ __align__(16) uint8_t bg_smem[];
-  uint8_t* sc = bg_smem;                              // [6][kBgTM][kBgS]
-  uint8_t* sh = bg_smem + 6 * kBgTM * kBgS;           // [5][kBgTN][kBgS]
+                  const uint8_t* __restrict__ cdig,
+                  const double* __restrict__ cscale,
+                  const uint8_t* __restrict__ hdig,
+                  const double* __restrict__ hscale,
+                  const double* __restrict__ hcorr, double* __restrict__ out,
+                  const int32_t* __restrict__ done, int tm, int tn,
+                  int per_xcd) {
+  // two buffers as two arrays: the compiler can tell the loads into one
+  // from the fragment reads of the other
+  __shared__ __align__(16) uint8_t buf0[kBdBuf];
+  __shared__ __align__(16) uint8_t buf1[kBdBuf];
   if (done && *done) return;
   int im, in;
   if (!oz_tile(tm, tn, per_xcd, &im, &in)) return;
@@ -798,37 +815,45 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   const int wn = (wave & 1) * 32;
   const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 6 * rows * ldk);
   const __amdgpu_buffer_rsrc_t rhd = buffer_of(hdig, 5 * lenb * ldk);
-  // staging: load j of a thread -> (plane, tile row, 16-byte segment) of
-  // index tid + 256 j in plane-major order
-  u32x4 cv[kBgC / 256], hv[kBgH / 256];
-  auto load = [&](int64_t k0) {
+  // lane L of a load instruction: tile row r0 + L/4, LDS chunk L%4, which
+  // holds global chunk (L%4) ^ ((L/16)%4) (r0 is a multiple of 16)
+  const int lrow = lane >> 2;
+  const int lq = (lane & 3) ^ ((lane >> 4) & 3);
+  // C: 6 planes x 8 instructions of 16 rows; H: 5 planes x 4; wave w takes
+  // instructions w, w + 4, ...  Byte offsets at k0 = 0, once: a row past
+  // the matrix gets kBdNoRow, which stays past the end of the digit planes
+  // for every k0 (their size is below kBdNoRow), so it reads zero.
+  int coff[12], hoff[5];
 #pragma unroll
-    for (int j = 0; j < kBgC / 256; ++j) {
-      const int x = tid + 256 * j;
-      const int i = x / (kBgTM * 4), r = (x / 4) % kBgTM, g = x % 4;
-      const int oc = m0 + r < rows ? static_cast<int>((i * rows + m0 + r) * ldk + k0 + 16 * g) : -1;
-      cv[j] = __builtin_amdgcn_raw_buffer_load_b128(rcd, oc, 0, 0);
+  for (int j = 0; j < 12; ++j) {
+    const int ins = wave + 4 * j;
+    const int i = ins >> 3, r0 = (ins & 7) * 16;
+    const int64_t gr = m0 + r0 + lrow;
+    coff[j] = gr < rows ? static_cast<int>((i * rows + gr) * ldk + 16 * lq) : kBdNoRow;
+  }
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const int ins = wave + 4 * j;
+    const int i = ins >> 2, r0 = (ins & 3) * 16;
+    const int64_t gn = n0 + r0 + lrow;
+    hoff[j] = gn < lenb ? static_cast<int>((i * lenb + gn) * ldk + 16 * lq) : kBdNoRow;
+  }
+  auto issue = [&](uint8_t* base, int k0) {
+#pragma unroll
+    for (int j = 0; j < 12; ++j) {
+      const int ins = wave + 4 * j;
+      const int i = ins >> 3, r0 = (ins & 7) * 16;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rcd, (lds_void_t*)(base + (i * kBgTM + r0) * kBgK), 16, coff[j] + k0,
+          0, 0, 0);
     }
 #pragma unroll
-    for (int j = 0; j < kBgH / 256; ++j) {
-      const int x = tid + 256 * j;
-      const int i = x / (kBgTN * 4), r = (x / 4) % kBgTN, g = x % 4;
-      const int oh = n0 + r < lenb ? static_cast<int>((i * lenb + n0 + r) * ldk + k0 + 16 * g) : -1;
-      hv[j] = __builtin_amdgcn_raw_buffer_load_b128(rhd, oh, 0, 0);
-    }
-  };
-  auto store = [&]() {
-#pragma unroll
-    for (int j = 0; j < kBgC / 256; ++j) {
-      const int x = tid + 256 * j;
-      const int i = x / (kBgTM * 4), r = (x / 4) % kBgTM, g = x % 4;
-      *reinterpret_cast<u32x4*>(sc + (i * kBgTM + r) * kBgS + 16 * g) = cv[j];
-    }
-#pragma unroll
-    for (int j = 0; j < kBgH / 256; ++j) {
-      const int x = tid + 256 * j;
-      const int i = x / (kBgTN * 4), r = (x / 4) % kBgTN, g = x % 4;
-      *reinterpret_cast<u32x4*>(sh + (i * kBgTN + r) * kBgS + 16 * g) = hv[j];
+    for (int j = 0; j < 5; ++j) {
+      const int ins = wave + 4 * j;
+      const int i = ins >> 2, r0 = (ins & 3) * 16;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(
+          rhd, (lds_void_t*)(base + 6 * kBgTM * kBgK + (i * kBgTN + r0) * kBgK),
+          16, hoff[j] + k0, 0, 0, 0);
     }
   };
   int4v acc[4][2][6];
@@ -838,42 +863,47 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int m = 0; m < 6; ++m) acc[a][b][m] = int4v{0, 0, 0, 0};
-
-  const int64_t nk = ldk / kBgK;
-  load(0);
-  store();
-  __syncthreads();
-  for (int64_t ks = 0; ks < nk; ++ks) {
-    if (ks + 1 < nk) load((ks + 1) * kBgK);
-    int4v a[4][6];
+  // both column tiles' Hbb fragments held, one row tile's C fragments at a
+  // time (64 operand registers)
+  auto compute = [&](const uint8_t* base) {
+    const uint8_t* sc = base;
+    const uint8_t* sh = base + 6 * kBgTM * kBgK;
+    const int q = lane >> 4;
+    int4v b[2][5];
 #pragma unroll
-    for (int x = 0; x < 4; ++x)
-#pragma unroll
-      for (int i = 0; i < 6; ++i)
-        a[x][i] = *reinterpret_cast<const int4v*>(
-            sc + (i * kBgTM + wm + 16 * x + (lane & 15)) * kBgS + 16 * (lane >> 4));
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      int4v b[5];
+    for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int i = 0; i < 5; ++i)
-        b[i] = *reinterpret_cast<const int4v*>(
-            sh + (i * kBgTN + wn + 16 * y + (lane & 15)) * kBgS + 16 * (lane >> 4));
+        b[y][i] = *reinterpret_cast<const int4v*>(
+            sh + i * kBgTN * kBgK + bd_off(wn + 16 * y + (lane & 15), q));
 #pragma unroll
-      for (int x = 0; x < 4; ++x)
+    for (int x = 0; x < 4; ++x) {
+      int4v a[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+        a[i] = *reinterpret_cast<const int4v*>(
+            sc + i * kBgTM * kBgK + bd_off(wm + 16 * x + (lane & 15), q));
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
 #pragma unroll
         for (int i = 0; i < 6; ++i)
 #pragma unroll
           for (int t = 0; t < 5; ++t)
             if (t <= i)
               acc[x][y][i - t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                  a[x][i], b[t], acc[x][y][i - t], 0, 0, 0);
+                  a[i], b[y][t], acc[x][y][i - t], 0, 0, 0);
     }
+  };
+  const int nk = static_cast<int>(ldk / kBgK);
+  issue(buf0, 0);
+  for (int ks = 0; ks < nk; ++ks) {
+    // step ks from one buffer; step ks + 1 loads into the other meanwhile
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    if (ks + 1 < nk) {
-      store();
-      __syncthreads();
-    }
+    uint8_t* cur = (ks & 1) ? buf1 : buf0;
+    uint8_t* nxt = (ks & 1) ? buf0 : buf1;
+    if (ks + 1 < nk) issue(nxt, (ks + 1) * kBgK);
+    compute(cur);
   }
   // output item e of lane l: M row 4 (l/16) + e, column l%16; rows 2 Ia
   // (real) and 2 Ia + 1 (imaginary) are items e and e + 1
@@ -906,8 +936,6 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
     }
 }
 
-constexpr size_t kBgLds = static_cast<size_t>(6 * kBgTM + 5 * kBgTN) * kBgS;
-
 }  // namespace
 
 int64_t beta_oz_ldk(int64_t lenb) { return (lenb + kBgK - 1) / kBgK * kBgK; }
@@ -918,7 +946,9 @@ int64_t beta_oz_workspace_bytes(int64_t lena, int64_t lenb) {
 }
 
 bool beta_oz_shape_ok(int64_t lena, int64_t lenb) {
-  const int64_t lim = int64_t(1) << 31;
+  // byte offsets into the digit planes stay below the out-of-range row
+  // offset of k_beta_gemm_oz
+  const int64_t lim = kBdNoRow - beta_oz_ldk(lenb);
   return lenb < 26214 && 6 * 2 * lena * beta_oz_ldk(lenb) < lim &&
          5 * lenb * beta_oz_ldk(lenb) < lim;
 }
@@ -942,7 +972,7 @@ int launch_beta_gemm_oz(int64_t lena, int64_t lenb, const void* hoz,
   const int tm = static_cast<int>((2 * lena + kBgTM - 1) / kBgTM);
   const int tn = static_cast<int>((lenb + kBgTN - 1) / kBgTN);
   const int per_xcd = (tm * tn + kBgXcds - 1) / kBgXcds;
-  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 256, kBgLds, st>>>(
+  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 256, 0, st>>>(
       lena, lenb, ldk, cdig, cscale, hdig, hscale, hcorr, out, done, tm, tn,
       per_xcd);
   FQEHIP_CHECK_LAUNCH("k_beta_gemm_oz");
