@@ -458,10 +458,12 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
     all-rows kernel), by npair + 1 columns (extra h1' column); the
     beta-beta GEMM is lena x lenb x lenb complex times real (bfloat16
     hi/lo copies of Hbb for the mixed engine, FP64 columns for FP64).
-    The Ozaki-digit engine moves the FP64 engine's bytes; its executed
-    int8 operations ("ops") are 20 digit products of the padded tiles: 64
-    live rows x 128 gathered rows per column (alpha rows) and 2 lena x
-    ldk x lenb (beta-beta GEMM, ldk = lenb rounded up to 64)."""
+    The Ozaki-digit engine moves the FP64 engine's bytes except for the
+    slot rows, which it writes and the reduction reads in 48 bits per
+    double (12 bytes per complex value); its executed int8 operations
+    ("ops") are 20 digit products of the padded tiles: 64 live rows x 128
+    gathered rows per column (alpha rows) and 2 lena x ldk x lenb
+    (beta-beta GEMM, ldk = lenb rounded up to 64)."""
     from fqe_amd import settings
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
@@ -477,9 +479,9 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
         return {
             "beta_gemm": {"bytes": 2 * 2 * n * esz + 5 * lenb * ldk,
                           "ops": 20 * 2.0 * (2 * lena) * ldk * lenb},
-            "alpha_rows": {"bytes": (nslot + 3) * 2 * n * esz,
+            "alpha_rows": {"bytes": nslot * 12 * n + 3 * 2 * n * esz,
                            "ops": 20 * 2.0 * 64 * 128 * 2 * n},
-            "slot_reduce": {"bytes": (nslot + 2) * 2 * n * esz},
+            "slot_reduce": {"bytes": nslot * 12 * n + 2 * 2 * n * esz},
         }
     return {
         "beta_gemm": {"bytes": 2 * 2 * n * esz + (2 * ldh * ldh * 2 if mixed

@@ -615,7 +615,7 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
                     const int32_t* fwa, const int32_t* sla, int nslot,
                     const double* coeff, const double* rowmax, void* btab,
-                    double* slots, double* rowacc, hipStream_t st,
+                    uint32_t* slots, double* rowacc, hipStream_t st,
                     const int32_t* done);
 int launch_row_absmax(int64_t lena, int64_t lenb, const double* coeff,
                       double* rowmax, hipStream_t st, const int32_t* done);
@@ -634,11 +634,11 @@ bool beta_gemm_f64_shape_ok(int64_t lena, int64_t lenb);
 int launch_beta_gemm_f64(int64_t lena, int64_t lenb, const double* hcol,
                          const double* coeff, double* out, hipStream_t st,
                          const int32_t* done);
-int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
-                       const void* slots, const void* rowacc, double ar,
-                       double ai, double br, double bi, const void* cin,
-                       void* out, double* acc, double* partials,
-                       hipStream_t st, const int32_t* done);
+int launch_slot_reduce(int precision, bool p48, int nslot, int64_t lena,
+                       int64_t lenb, const void* slots, const void* rowacc,
+                       double ar, double ai, double br, double bi,
+                       const void* cin, void* out, double* acc,
+                       double* partials, hipStream_t st, const int32_t* done);
 
 // Declared in gemm_bf16.hip.
 int launch_gemm_bf16x3(int m, int k, const double* a, int64_t ncol,
@@ -907,7 +907,7 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
     return launch_split_oz(s->norb, s->npair, s->lena, s->lenb, op->a,
                            s->ptab_a, s->ptab_b, s->fwd_a, s->slot_a, s->nslot,
                            static_cast<const double*>(d_coeff), rowmax, btab,
-                           reinterpret_cast<double*>(ws),
+                           reinterpret_cast<uint32_t*>(ws),
                            static_cast<double*>(rowacc), st, d_done);
   }
   if (precision == FQEHIP_C128 && s->hbb_f64 != nullptr) {
@@ -955,7 +955,9 @@ extern "C" int fqehip_apply_slot_reduce(int precision, const fqehip_sector* s,
   const int64_t n = s->lena * s->lenb;
   const char* ws = static_cast<const char*>(d_workspace);
   const void* rowacc = ws + static_cast<int64_t>(s->nslot) * 2 * n * elem_size(precision);
-  return launch_slot_reduce(precision, s->nslot, s->lena, s->lenb, ws, rowacc,
+  // the Ozaki-digit row kernel writes the slot rows in 48 bits per double
+  return launch_slot_reduce(precision, ozaki_split(precision, s), s->nslot,
+                            s->lena, s->lenb, ws, rowacc,
                             alpha_re, alpha_im, beta_re, beta_im, d_cin, d_out,
                             d_acc, d_norm_partials, as_stream(stream), d_done);
 }

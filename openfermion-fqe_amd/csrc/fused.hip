@@ -1204,42 +1204,54 @@ k_split_cols_f64(int norb, int compressed, int npair, int64_t lena,
 // ---------------------------------------------------------------------------
 // Element idx = (ia, ib) of a slot reduction: rowacc + the nslot slots in
 // FP64, out = alpha * that + beta * cin, acc += the FP64 value; returns
-// |out|^2 (shared by k_slot_reduce and k_slot_reduce_group).
-template <typename T>
+// |out|^2 (shared by k_slot_reduce and k_slot_reduce_group).  P48: the
+// slots are complex128 values packed in 48 bits per double
+// (stream_store_p48).
+template <typename T, bool P48 = false>
 __device__ __forceinline__ double slot_element(
     int64_t idx, int64_t ia, int64_t ib, int nslot, int64_t lenb,
-    const T* __restrict__ slots, const T* __restrict__ rowacc,
+    const void* __restrict__ slotv, const T* __restrict__ rowacc,
     double alpha_re, double alpha_im, double beta_re, double beta_im,
     const T* cin, T* out, double* __restrict__ acc) {
   double re = static_cast<double>(rowacc[2 * idx]);
   double im = static_cast<double>(rowacc[2 * idx + 1]);
-  const T* s = slots + (ia * nslot * lenb + ib) * 2;
-  // complex64 slots: eight loads in flight per thread (complex128 slots
-  // keep one, which measured faster); the FP64 sum keeps slot order
-  constexpr int kU = sizeof(T) == 4 ? 8 : 1;
-  int k = 0;
-  typedef T t2v __attribute__((ext_vector_type(2)));
-  for (; k + kU <= nslot; k += kU) {
-    T vr[kU], vi[kU];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const t2v v = __builtin_nontemporal_load(
-          reinterpret_cast<const t2v*>(s + 2 * lenb * u));
-      vr[u] = v[0];
-      vi[u] = v[1];
+  if constexpr (P48) {
+    const uint32_t* s = static_cast<const uint32_t*>(slotv) + (ia * nslot * lenb + ib) * 3;
+    for (int k = 0; k < nslot; ++k) {
+      const u3v v = stream_load_p48(s);
+      re += p48_re(v);
+      im += p48_im(v);
+      s += 3 * lenb;
     }
+  } else {
+    const T* s = static_cast<const T*>(slotv) + (ia * nslot * lenb + ib) * 2;
+    // complex64 slots: eight loads in flight per thread (complex128 slots
+    // keep one, which measured faster); the FP64 sum keeps slot order
+    constexpr int kU = sizeof(T) == 4 ? 8 : 1;
+    typedef T t2v __attribute__((ext_vector_type(2)));
+    int k = 0;
+    for (; k + kU <= nslot; k += kU) {
+      T vr[kU], vi[kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      re += static_cast<double>(vr[u]);
-      im += static_cast<double>(vi[u]);
+      for (int u = 0; u < kU; ++u) {
+        const t2v v = __builtin_nontemporal_load(
+            reinterpret_cast<const t2v*>(s + 2 * lenb * u));
+        vr[u] = v[0];
+        vi[u] = v[1];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        re += static_cast<double>(vr[u]);
+        im += static_cast<double>(vi[u]);
+      }
+      s += 2 * lenb * kU;
     }
-    s += 2 * lenb * kU;
-  }
-  for (; k < nslot; ++k) {
-    const t2v v = __builtin_nontemporal_load(reinterpret_cast<const t2v*>(s));
-    re += static_cast<double>(v[0]);
-    im += static_cast<double>(v[1]);
-    s += 2 * lenb;
+    for (; k < nslot; ++k) {
+      const t2v v = __builtin_nontemporal_load(reinterpret_cast<const t2v*>(s));
+      re += static_cast<double>(v[0]);
+      im += static_cast<double>(v[1]);
+      s += 2 * lenb;
+    }
   }
   double vr = alpha_re * re - alpha_im * im;
   double vi = alpha_re * im + alpha_im * re;
@@ -1260,10 +1272,10 @@ __device__ __forceinline__ double slot_element(
   return static_cast<double>(orr) * orr + static_cast<double>(oi) * oi;
 }
 
-template <typename T>
+template <typename T, bool P48>
 __global__ void __launch_bounds__(kThreads)
 k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
-              const T* __restrict__ slots, const T* __restrict__ rowacc,
+              const void* __restrict__ slots, const T* __restrict__ rowacc,
               double alpha_re, double alpha_im, double beta_re, double beta_im,
               const T* cin, T* out, double* __restrict__ acc,
               double* __restrict__ partials, const int32_t* __restrict__ done) {
@@ -1276,8 +1288,9 @@ k_slot_reduce(int nslot, int64_t lena, int64_t lenb, double inv_lenb,
     int64_t ia = static_cast<int64_t>(static_cast<double>(idx) * inv_lenb);
     int64_t ib = idx - ia * lenb;
     if (ib < 0) { --ia; ib += lenb; } else if (ib >= lenb) { ++ia; ib -= lenb; }
-    sq = slot_element<T>(idx, ia, ib, nslot, lenb, slots, rowacc, alpha_re,
-                         alpha_im, beta_re, beta_im, cin, out, acc);
+    sq = slot_element<T, P48>(idx, ia, ib, nslot, lenb, slots, rowacc,
+                              alpha_re, alpha_im, beta_re, beta_im, cin, out,
+                              acc);
   }
   if (partials) {
     for (int off = 32; off > 0; off >>= 1) sq += __shfl_down(sq, off, 64);
@@ -1574,23 +1587,29 @@ int launch_fused(int precision, int norb, int npair, int64_t lena,
   return FQEHIP_OK;
 }
 
-int launch_slot_reduce(int precision, int nslot, int64_t lena, int64_t lenb,
-                       const void* slots, const void* rowacc, double ar,
-                       double ai, double br, double bi, const void* cin,
-                       void* out, double* acc, double* partials,
-                       hipStream_t st, const int32_t* done) {
+int launch_slot_reduce(int precision, bool p48, int nslot, int64_t lena,
+                       int64_t lenb, const void* slots, const void* rowacc,
+                       double ar, double ai, double br, double bi,
+                       const void* cin, void* out, double* acc,
+                       double* partials, hipStream_t st, const int32_t* done) {
   const int64_t n = lena * lenb;
   if (n == 0) return FQEHIP_OK;
   const unsigned nb = blocks_for(n, kThreads);
-  if (precision == FQEHIP_C128) {
-    k_slot_reduce<double><<<nb, kThreads, 0, st>>>(
-        nslot, lena, lenb, 1.0 / lenb, static_cast<const double*>(slots),
+  if (precision == FQEHIP_C128 && p48) {
+    k_slot_reduce<double, true><<<nb, kThreads, 0, st>>>(
+        nslot, lena, lenb, 1.0 / lenb, slots,
+        static_cast<const double*>(rowacc), ar, ai, br, bi,
+        static_cast<const double*>(cin), static_cast<double*>(out), acc,
+        partials, done);
+  } else if (precision == FQEHIP_C128) {
+    k_slot_reduce<double, false><<<nb, kThreads, 0, st>>>(
+        nslot, lena, lenb, 1.0 / lenb, slots,
         static_cast<const double*>(rowacc), ar, ai, br, bi,
         static_cast<const double*>(cin), static_cast<double*>(out), acc,
         partials, done);
   } else {
-    k_slot_reduce<float><<<nb, kThreads, 0, st>>>(
-        nslot, lena, lenb, 1.0 / lenb, static_cast<const float*>(slots),
+    k_slot_reduce<float, false><<<nb, kThreads, 0, st>>>(
+        nslot, lena, lenb, 1.0 / lenb, slots,
         static_cast<const float*>(rowacc), ar, ai, br, bi,
         static_cast<const float*>(cin), static_cast<float*>(out), acc,
         partials, done);

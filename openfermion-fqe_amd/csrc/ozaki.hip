@@ -1,6 +1,7 @@
 // Ozaki-digit variant of the complex128 split apply: the same operands,
-// layouts and scatter as the FP64 split path (complex128 C, FP64 slot rows
-// and row accumulators), with the pair contraction Y = A' D' of the alpha
+// layouts and scatter as the FP64 split path (complex128 C, FP64 row
+// accumulators; the slot rows in 48 bits per double, 12 bytes per complex
+// value, stream_store_p48), with the pair contraction Y = A' D' of the alpha
 // row kernel computed as exact integer products of 8-bit digits on the
 // matrix cores (v_mfma_i32_16x16x64_i8, int32 accumulation) instead of
 // FP64 matrix-core products.
@@ -252,7 +253,7 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
                 const double* __restrict__ pzs,
                 const int32_t* __restrict__ fwa, const int32_t* __restrict__ sla,
                 int nslot, const double* __restrict__ coeff,
-                const double* __restrict__ rowmax, double* __restrict__ slots,
+                const double* __restrict__ rowmax, uint32_t* __restrict__ slots,
                 double* __restrict__ rowacc, const int32_t* __restrict__ done) {
   extern __shared__ __align__(16) char smem[];
   if (done && *done) return;
@@ -541,7 +542,7 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
             } else if (po >= 0) {
               const int64_t sb = rt.sbase[po];
               const double sg = static_cast<double>(rt.ssgn[po]);
-              stream_store2(slots + sb + 2 * jb, sg * y[0], sg * y[1]);
+              stream_store_p48(slots + 3 * ((sb >> 1) + jb), sg * y[0], sg * y[1]);
             }
           }
         }
@@ -585,7 +586,7 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
                     const int32_t* fwa, const int32_t* sla, int nslot,
                     const double* coeff, const double* rowmax, void* btab,
-                    double* slots, double* rowacc, hipStream_t st,
+                    uint32_t* slots, double* rowacc, hipStream_t st,
                     const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   if (!split_oz_supported(norb, npair, lena, lenb)) {

@@ -53,6 +53,34 @@ __device__ __forceinline__ void stream_store2(float* p, float x, float y) {
   __builtin_nontemporal_store(f2v{x, y}, reinterpret_cast<f2v*>(p));
 }
 
+// Slot rows of the Ozaki-digit engine in 48 bits per double: sign,
+// exponent and the top 36 mantissa bits, rounded to nearest (ties away
+// from zero) at bit 16, so the relative rounding error is at most 2^-37.
+// One complex value is three words {hi(re), hi(im), lo(re) | lo(im) << 16}
+// (hi: bits 63..32, lo: bits 31..16), 12 bytes at a 4-byte alignment.
+typedef uint32_t u3v __attribute__((ext_vector_type(3)));
+typedef u3v u3v_a4 __attribute__((aligned(4)));
+__device__ __forceinline__ void stream_store_p48(uint32_t* p, double x,
+                                                 double y) {
+  const uint64_t bx = static_cast<uint64_t>(__double_as_longlong(x)) + 0x8000u;
+  const uint64_t by = static_cast<uint64_t>(__double_as_longlong(y)) + 0x8000u;
+  const u3v v{static_cast<uint32_t>(bx >> 32), static_cast<uint32_t>(by >> 32),
+              static_cast<uint32_t>((bx >> 16) & 0xffffu) |
+                  (static_cast<uint32_t>(by >> 16) << 16)};
+  __builtin_nontemporal_store(v, reinterpret_cast<u3v_a4*>(p));
+}
+__device__ __forceinline__ u3v stream_load_p48(const uint32_t* p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const u3v_a4*>(p));
+}
+__device__ __forceinline__ double p48_re(u3v v) {
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<uint64_t>(v[0]) << 32) | (static_cast<uint64_t>(v[2] & 0xffffu) << 16)));
+}
+__device__ __forceinline__ double p48_im(u3v v) {
+  return __longlong_as_double(static_cast<long long>(
+      (static_cast<uint64_t>(v[1]) << 32) | (static_cast<uint64_t>(v[2] >> 16) << 16)));
+}
+
 constexpr int kRowTabs = 128;              // per-row LDS tables (>= npair + 1)
 
 struct RowTables {
