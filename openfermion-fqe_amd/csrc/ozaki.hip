@@ -227,6 +227,8 @@ k_pair_digits_oz(int npair, const double* __restrict__ amat,
 
 constexpr int kRowThreads = 512;           // 8 waves, 16 columns each
 constexpr int kRowCols = 128;              // columns per chunk
+constexpr int64_t kDstNone = -1;           // padding row: no store
+constexpr int64_t kDstRowacc = -2;         // folded row: into rowacc
 
 // Alpha rows of the split apply (see k_split_cols_f64 in fused.hip for the
 // operands), one alpha string per block at a time.  Set-up: the live pair
@@ -266,9 +268,12 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
   double* gsa = reinterpret_cast<double*>(adig + kOzADig * kLiveRows * kOzARow);
   uint32_t* goff = reinterpret_cast<uint32_t*>(gsa + kOzKPad);  // [kOzKPad]
   double* drow = reinterpret_cast<double*>(goff + kOzKPad);     // [kOzKPad]
-  double* rscale = drow + kOzKPad;          // [kLiveRows] 2^(32 - ea - ed)
+  double* rscale = drow + kOzKPad;          // [kLiveRows] +-2^(32 - ea - ed)
   double* corr = rscale + kLiveRows;        // [kLiveRows] offset correction
-  double* red = corr + kLiveRows;           // [8] block scratch
+  // [kLiveRows] destination of a live row: word offset of its slot row,
+  // kDstRowacc (the folded row) or kDstNone (padding)
+  int64_t* rdst = reinterpret_cast<int64_t*>(corr + kLiveRows);
+  double* red = reinterpret_cast<double*>(rdst + kLiveRows);  // [8] block scratch
   RowTables rt;
   rt.sbase = reinterpret_cast<int64_t*>(red + 8);
   rt.aoff = reinterpret_cast<int32_t*>(rt.sbase + kRowTabs);
@@ -391,10 +396,12 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
           dst[1] = w[t][1];
         }
         if ((tid & 3) == 0) {
-          // Y = 2^(32 - ea - ed) t + 2^-(ea + ed) corr (powers of two: exact)
-          const double sc = p >= 0 ? ldexp(1.0, -(pea[p] + ed)) : 0.0;
+          // Y = 2^(32 - ea - ed) t + 2^-(ea + ed) corr (powers of two: exact),
+          // times the sign of the scatter to the slot row
+          const double sc = p >= 0 ? ldexp(1.0, -(pea[p] + ed)) * rt.ssgn[p] : 0.0;
           corr[m] = p >= 0 ? kOffsetUnit * pzs[p] * sc : 0.0;
           rscale[m] = 4294967296.0 * sc;
+          rdst[m] = p >= 0 ? 3 * (rt.sbase[p] >> 1) : kDstNone;
         }
       }
     } else if (wave == 6) {
@@ -421,6 +428,7 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
           const double sc = ldexp(1.0, -(ea + ed));
           corr[m] = kOffsetUnit * zsum * sc;
           rscale[m] = 4294967296.0 * sc;
+          rdst[m] = kDstRowacc;
         }
       }
     }
@@ -532,17 +540,15 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
               t = fma(t, 256.0, static_cast<double>(acc[c][0][e]));
               y[c] = fma(t, rscale[row], corr[row]);
             }
-            const int po = lpair[row];
-            if (po == npair) {
+            const int64_t dst = rdst[row];
+            if (dst >= 0) {
+              stream_store_p48(slots + dst + 3 * jb, y[0], y[1]);
+            } else if (dst == kDstRowacc) {
               double2* d = reinterpret_cast<double2*>(rowacc + ia * lenb * 2 + 2 * jb);
               double2 o = *d;
               o.x += y[0];
               o.y += y[1];
               *d = o;
-            } else if (po >= 0) {
-              const int64_t sb = rt.sbase[po];
-              const double sg = static_cast<double>(rt.ssgn[po]);
-              stream_store_p48(slots + 3 * ((sb >> 1) + jb), sg * y[0], sg * y[1]);
             }
           }
         }
@@ -555,7 +561,7 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
 size_t split_rows_oz_lds_bytes(bool row_lds, int64_t lenb) {
   return static_cast<size_t>(kOzADig) * kLiveRows * kOzARow +
          kOzKPad * (sizeof(double) + sizeof(uint32_t)) +
-         (kOzKPad + 2 * kLiveRows + 8) * sizeof(double) +
+         (kOzKPad + 3 * kLiveRows + 8) * sizeof(double) +
          kRowTabs * (sizeof(int64_t) + 3 * sizeof(int32_t)) +
          kLiveRows * sizeof(int32_t) +
          (row_lds ? static_cast<size_t>(lenb + 1) * 16 : 0);
