@@ -615,8 +615,8 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
                     const int32_t* fwa, const int32_t* sla, int nslot,
                     const double* coeff, const double* rowmax, void* btab,
-                    uint32_t* slots, double* rowacc, hipStream_t st,
-                    const int32_t* done);
+                    uint32_t* slots, double* rowacc, bool build_tables,
+                    hipStream_t st, const int32_t* done);
 int launch_row_absmax(int64_t lena, int64_t lenb, const double* coeff,
                       double* rowmax, hipStream_t st, const int32_t* done);
 bool beta_oz_shape_ok(int64_t lena, int64_t lenb);
@@ -868,11 +868,13 @@ extern "C" int fqehip_fused_supported(int precision, const fqehip_sector* s,
   return fused_supported(precision, s->npair, s->lena, s->lenb);
 }
 
-extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
-                                       const fqehip_operator* op,
-                                       const void* d_coeff, void* d_workspace,
-                                       const int32_t* d_done, void* stream) {
-  clear_error();
+namespace fqehip {
+namespace {
+
+// fqehip_apply_fused_rows; reuse_tables: see apply_reusing_tables.
+int fused_rows(int precision, const fqehip_sector* s, const fqehip_operator* op,
+               const void* d_coeff, void* d_workspace, const int32_t* d_done,
+               void* stream, bool reuse_tables) {
   if (int rc = check_sector_op(precision, s, op)) return rc;
   FQEHIP_CHECK_ARG(fqehip_fused_supported(precision, s, op),
                    "the fused path does not support this sector/operator");
@@ -908,7 +910,8 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
                            s->ptab_a, s->ptab_b, s->fwd_a, s->slot_a, s->nslot,
                            static_cast<const double*>(d_coeff), rowmax, btab,
                            reinterpret_cast<uint32_t*>(ws),
-                           static_cast<double*>(rowacc), st, d_done);
+                           static_cast<double*>(rowacc), !reuse_tables, st,
+                           d_done);
   }
   if (precision == FQEHIP_C128 && s->hbb_f64 != nullptr) {
     // rowacc <- C Hbb^T (FP64), then the live alpha rows add their part
@@ -938,6 +941,18 @@ extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
                       s->ptab_a, s->ptab_b, s->fwd_a, s->fwd_b, s->slot_a,
                       s->nslot, d_coeff, ws, rowacc, split, s->tile, st,
                       d_done);
+}
+
+}  // namespace
+}  // namespace fqehip
+
+extern "C" int fqehip_apply_fused_rows(int precision, const fqehip_sector* s,
+                                       const fqehip_operator* op,
+                                       const void* d_coeff, void* d_workspace,
+                                       const int32_t* d_done, void* stream) {
+  clear_error();
+  return fused_rows(precision, s, op, d_coeff, d_workspace, d_done, stream,
+                    false);
 }
 
 extern "C" int fqehip_apply_slot_reduce(int precision, const fqehip_sector* s,
@@ -972,14 +987,17 @@ extern "C" int64_t fqehip_apply_workspace_bytes(int precision,
   return unfused > fused ? unfused : fused;
 }
 
-extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
-                            const fqehip_operator* op, const void* d_coeff,
-                            double alpha_re, double alpha_im, double beta_re,
-                            double beta_im, const void* d_cin, void* d_out,
-                            double* d_acc, double* d_norm_partials,
-                            const int32_t* d_done, void* d_workspace,
-                            void* stream) {
-  clear_error();
+// fqehip_apply.  reuse_tables: the Ozaki-digit split apply takes the beta
+// gather table and the pair rows' digits from the workspace as an earlier
+// apply of the same sector and operator with the same workspace left them
+// (they depend on neither the coefficients nor alpha/beta), instead of
+// rebuilding them: the later terms of a Taylor series.
+int fqehip::apply_reusing_tables(
+    int precision, const fqehip_sector* s, const fqehip_operator* op,
+    const void* d_coeff, double alpha_re, double alpha_im, double beta_re,
+    double beta_im, const void* d_cin, void* d_out, double* d_acc,
+    double* d_norm_partials, const int32_t* d_done, void* d_workspace,
+    void* stream, bool reuse_tables) {
   if (int rc = check_sector_op(precision, s, op)) return rc;
   FQEHIP_CHECK_ARG(d_coeff && d_out, "NULL coefficient pointer");
   if (op->h1)
@@ -991,8 +1009,8 @@ extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
   const int64_t n = s->lena * s->lenb;
   char* ws = static_cast<char*>(d_workspace);
   if (fqehip_fused_supported(precision, s, op)) {
-    int rc = fqehip_apply_fused_rows(precision, s, op, d_coeff, d_workspace,
-                                     d_done, stream);
+    int rc = fused_rows(precision, s, op, d_coeff, d_workspace, d_done, stream,
+                        reuse_tables);
     if (rc) return rc;
     return fqehip_apply_slot_reduce(precision, s, d_workspace, alpha_re,
                                     alpha_im, beta_re, beta_im, d_cin, d_out,
@@ -1014,6 +1032,20 @@ extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
   return fqehip_scatter(precision, s->npair, s->lena, s->lenb, s->ptab_a,
                         s->ptab_b, y, alpha_re, alpha_im, beta_re, beta_im,
                         d_cin, d_out, d_acc, d_norm_partials, d_done, stream);
+}
+
+extern "C" int fqehip_apply(int precision, const fqehip_sector* s,
+                            const fqehip_operator* op, const void* d_coeff,
+                            double alpha_re, double alpha_im, double beta_re,
+                            double beta_im, const void* d_cin, void* d_out,
+                            double* d_acc, double* d_norm_partials,
+                            const int32_t* d_done, void* d_workspace,
+                            void* stream) {
+  clear_error();
+  return apply_reusing_tables(precision, s, op, d_coeff, alpha_re, alpha_im,
+                              beta_re, beta_im, d_cin, d_out, d_acc,
+                              d_norm_partials, d_done, d_workspace, stream,
+                              false);
 }
 
 extern "C" int fqehip_taylor(int precision, const fqehip_sector* s,
@@ -1050,9 +1082,9 @@ extern "C" int fqehip_taylor(int precision, const fqehip_sector* s,
       cur = nxt;
       nxt = t;
     } else {
-      rc = fqehip_apply(precision, s, op, d_term, ar, ai, 0.0, 0.0, nullptr,
-                        d_term, d_result, d_partials, done, d_workspace,
-                        stream);
+      rc = apply_reusing_tables(precision, s, op, d_term, ar, ai, 0.0, 0.0,
+                                nullptr, d_term, d_result, d_partials, done,
+                                d_workspace, stream, order > 1);
     }
     if (rc) return rc;
     rc = fqehip_norm_finalize(d_partials, np, d_norm, accuracy, order, done,

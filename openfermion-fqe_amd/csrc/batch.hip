@@ -80,10 +80,13 @@ int enqueue_item(int precision, const fqehip_batch_item& it, double time,
     *nodes += 3;
   }
   for (int k = 1; k <= nterms; ++k) {
-    rc = fqehip_apply(precision, &it.sector, &it.op, it.d_term, 0.0,
-                      -time / static_cast<double>(k), 0.0, 0.0, nullptr,
-                      it.d_term, it.d_result, nullptr, nullptr,
-                      it.d_workspace, sv);
+    // after the sector's first apply its workspace holds the operator's
+    // tables
+    rc = apply_reusing_tables(precision, &it.sector, &it.op, it.d_term, 0.0,
+                              -time / static_cast<double>(k), 0.0, 0.0,
+                              nullptr, it.d_term, it.d_result, nullptr,
+                              nullptr, it.d_workspace, sv,
+                              it.d_apply_out != nullptr || k > 1);
     if (rc) return rc;
     *nodes += 3;
   }

@@ -139,4 +139,15 @@ struct GroupSector {
   void* rowacc;                 // workspace: row accumulators
 };
 
+// fqehip_apply that, with reuse_tables, takes the Ozaki-digit split apply's
+// operator tables from the workspace as an earlier apply of the same sector
+// and operator with the same workspace built them (apply.hip).
+int apply_reusing_tables(int precision, const fqehip_sector* s,
+                         const fqehip_operator* op, const void* d_coeff,
+                         double alpha_re, double alpha_im, double beta_re,
+                         double beta_im, const void* d_cin, void* d_out,
+                         double* d_acc, double* d_norm_partials,
+                         const int32_t* d_done, void* d_workspace,
+                         void* stream, bool reuse_tables);
+
 }  // namespace fqehip

@@ -588,12 +588,14 @@ int64_t split_oz_table_bytes(int64_t lenb) {
   return lenb * kOzKPad * 4 + kPairDigitBytes;
 }
 
+// build_tables false: btab already holds the beta table and the pair rows'
+// digits of this sector and operator (an earlier call with the same btab).
 int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
                     const double* a, const int32_t* pta, const int32_t* ptb,
                     const int32_t* fwa, const int32_t* sla, int nslot,
                     const double* coeff, const double* rowmax, void* btab,
-                    uint32_t* slots, double* rowacc, hipStream_t st,
-                    const int32_t* done) {
+                    uint32_t* slots, double* rowacc, bool build_tables,
+                    hipStream_t st, const int32_t* done) {
   if (lena == 0 || lenb == 0) return FQEHIP_OK;
   if (!split_oz_supported(norb, npair, lena, lenb)) {
     set_error("the Ozaki split apply needs npair + 1 <= %d, norb^2/4 + 1 <= %d, "
@@ -605,16 +607,18 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
   uint8_t* pdig = reinterpret_cast<uint8_t*>(tab + lenb * kOzKPad);
   int32_t* pea = reinterpret_cast<int32_t*>(pdig + kOzKPad * kOzADig * kOzKPad);
   double* pzs = reinterpret_cast<double*>(pea + kOzKPad);
-  k_pair_digits_oz<<<(npair + 63) / 64, 256, 0, st>>>(npair, a, pdig, pea, pzs, done);
-  FQEHIP_CHECK_LAUNCH("k_pair_digits_oz");
   const bool row_lds = split_rows_oz_lds_bytes(true, lenb) <= kLdsLimit;
-  const int64_t entries = lenb * kOzKPad;
-  const unsigned tgrid = static_cast<unsigned>(std::min<int64_t>((entries + 255) / 256, 4096));
-  // absent beta excitations: the zero entry after the LDS row, or past the
-  // end of the row in global memory
-  const uint32_t none = row_lds ? static_cast<uint32_t>(16 * lenb) : kNoCol;
-  k_beta_table_oz<<<tgrid, 256, 0, st>>>(npair, lenb, ptb, none, tab, done);
-  FQEHIP_CHECK_LAUNCH("k_beta_table_oz");
+  if (build_tables) {
+    k_pair_digits_oz<<<(npair + 63) / 64, 256, 0, st>>>(npair, a, pdig, pea, pzs, done);
+    FQEHIP_CHECK_LAUNCH("k_pair_digits_oz");
+    const int64_t entries = lenb * kOzKPad;
+    const unsigned tgrid = static_cast<unsigned>(std::min<int64_t>((entries + 255) / 256, 4096));
+    // absent beta excitations: the zero entry after the LDS row, or past the
+    // end of the row in global memory
+    const uint32_t none = row_lds ? static_cast<uint32_t>(16 * lenb) : kNoCol;
+    k_beta_table_oz<<<tgrid, 256, 0, st>>>(npair, lenb, ptb, none, tab, done);
+    FQEHIP_CHECK_LAUNCH("k_beta_table_oz");
+  }
   const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
   const unsigned grid = static_cast<unsigned>(lena < 65536 ? lena : 65536);
   const size_t lds = split_rows_oz_lds_bytes(row_lds, lenb);
