@@ -246,10 +246,14 @@ constexpr int64_t kDstRowacc = -2;         // folded row: into rowacc
 // table words load during the products.  i8 MFMA layouts: lane l holds
 // A[row l%16][k 16 (l/16) + j] and B[k 16 (l/16) + j][col l%16] (byte j of
 // its four words); output item e of lane l is [row 4 (l/16) + e][col l%16].
+// Work items: alpha strings 0 .. nwhole - 1 whole (in XCD order), then each
+// of the remaining strings as two items, its first and second half of the
+// column chunks (so a last, partial round of strings over the CUs takes
+// half as long).
 template <bool ROW_LDS>
 __global__ void __launch_bounds__(kRowThreads)
 k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
-                int64_t lenb, const double* __restrict__ amat,
+                int64_t nwhole, int64_t lenb, const double* __restrict__ amat,
                 const int32_t* __restrict__ pta, const uint32_t* __restrict__ btab,
                 const uint8_t* __restrict__ pdig, const int32_t* __restrict__ pea,
                 const double* __restrict__ pzs,
@@ -303,10 +307,18 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
       for (int v = 0; v < 4; ++v)
         tw[s][v] = __builtin_amdgcn_raw_buffer_load_b128(rtb, base + 256 * s + 16 * v, 0, 0);
   };
-  load_table(0);
+  const int64_t nitems = nwhole + 2 * (lena - nwhole);
+  const int hchunks = (nchunks + 1) / 2;    // chunks of a first half
+  auto first_chunk = [&](int64_t it) {
+    return (it >= nwhole && ((it - nwhole) & 1)) ? hchunks : 0;
+  };
+  load_table(first_chunk(blockIdx.x));
 
-  for (int64_t it = blockIdx.x; it < lena; it += gridDim.x) {
-    const int64_t ia = xcd_order(it, lena);
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const int64_t ia = it < nwhole ? xcd_order(it, nwhole) : nwhole + ((it - nwhole) >> 1);
+    const int c0 = first_chunk(it);
+    const int c1 = (it < nwhole || c0 > 0) ? nchunks : hchunks;
+    const int next_c0 = first_chunk(it + gridDim.x);
     load_row_tables<double>(ia, K, lena, lenb, nslot, pta, fwa, sla, rt, tid,
                             kRowThreads, K);
     if (ROW_LDS) {
@@ -438,7 +450,7 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
     const __amdgpu_buffer_rsrc_t rrow = buffer_of(coeff + ia * lenb * 2, lenb * 16);
     const double dsc2 = 2.0 * dsc;
 
-    for (int chunk = 0; chunk < nchunks; ++chunk) {
+    for (int chunk = c0; chunk < c1; ++chunk) {
       const int64_t jb = static_cast<int64_t>(chunk) * kRowCols + wave * 16 + cl;
       const bool colok = jb < lenb;
       // padding columns gather column 0 (valid memory) and store nothing
@@ -494,8 +506,8 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
         }
         asm volatile("" ::: "memory");
       }
-      // the next chunk's table (chunk 0 again for the next alpha string)
-      load_table(chunk + 1 < nchunks ? chunk + 1 : 0);
+      // the next chunk's table (the next item's first chunk after the last)
+      load_table(chunk + 1 < c1 ? chunk + 1 : next_c0);
 #pragma unroll 1
       for (int r = 0; r < kLiveRows / 16; ++r) {
         int4v acc[2][6];
@@ -620,16 +632,28 @@ int launch_split_oz(int norb, int npair, int64_t lena, int64_t lenb,
     FQEHIP_CHECK_LAUNCH("k_beta_table_oz");
   }
   const int compressed = npair == norb * (norb + 1) / 2 ? 1 : 0;
-  const unsigned grid = static_cast<unsigned>(lena < 65536 ? lena : 65536);
+  // one workgroup per CU at a time: strings beyond the last full round over
+  // the CUs run as column halves when they fill at most half a round
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+    set_error("launch_split_oz: no device attributes");
+    return FQEHIP_ERR_HIP;
+  }
+  const int64_t rem = ncu > 0 ? lena % ncu : 0;
+  const int64_t nchunks = (lenb + kRowCols - 1) / kRowCols;
+  const int64_t nwhole = (nchunks >= 2 && 2 * rem <= ncu) ? lena - rem : lena;
+  const int64_t nitems = nwhole + 2 * (lena - nwhole);
+  const unsigned grid = static_cast<unsigned>(nitems < 65536 ? nitems : 65536);
   const size_t lds = split_rows_oz_lds_bytes(row_lds, lenb);
   if (row_lds)
     k_split_rows_oz<true><<<grid, kRowThreads, lds, st>>>(
-        norb, compressed, npair, lena, lenb, a, pta, tab, pdig, pea, pzs, fwa,
-        sla, nslot, coeff, rowmax, slots, rowacc, done);
+        norb, compressed, npair, lena, nwhole, lenb, a, pta, tab, pdig, pea,
+        pzs, fwa, sla, nslot, coeff, rowmax, slots, rowacc, done);
   else
     k_split_rows_oz<false><<<grid, kRowThreads, lds, st>>>(
-        norb, compressed, npair, lena, lenb, a, pta, tab, pdig, pea, pzs, fwa,
-        sla, nslot, coeff, rowmax, slots, rowacc, done);
+        norb, compressed, npair, lena, nwhole, lenb, a, pta, tab, pdig, pea,
+        pzs, fwa, sla, nslot, coeff, rowmax, slots, rowacc, done);
   FQEHIP_CHECK_LAUNCH("k_split_rows_oz");
   return FQEHIP_OK;
 }
