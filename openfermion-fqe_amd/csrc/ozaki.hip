@@ -815,9 +815,12 @@ k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
 // per CU) and filled by direct buffer-to-LDS loads (buffer_load_dwordx4
 // ... lds: lane L of an instruction writes bytes 16 L .. 16 L + 15 past the
 // instruction's LDS base), so no staging registers: one instruction moves
-// 16 rows x 64 bytes of one digit plane.  Rows are 64 bytes, unpadded; the
-// four 16-byte chunks of row r sit XOR-swizzled by (r / 4) % 4, so the 16
-// lanes of a fragment read (rows r..r+15 at one chunk) hit distinct banks.
+// 16 rows x 64 bytes of one digit plane.  Rows are 64 bytes (16 banks),
+// unpadded; the four 16-byte chunks of row r sit XOR-swizzled by (r / 2) % 4,
+// so each 8 lanes of a fragment read (rows r..r+7 at one chunk: even and odd
+// rows on the two bank halves, the four rows of each parity on four chunks)
+// cover the 32 banks once (keyed by (r / 4) % 4, rows r and r + 2 shared
+// banks: 1.5-2% slower).
 // Step ks + 1's loads go out after the barrier of step ks and land in the
 // other buffer while step ks's MFMAs run; one wait and one barrier per
 // step.
@@ -825,7 +828,7 @@ constexpr int kBdBuf = (6 * kBgTM + 5 * kBgTN) * kBgK;   // bytes per buffer
 constexpr int kBdNoRow = 0x7ff00000;        // offset of a row past the matrix
 
 __device__ __forceinline__ int bd_off(int row, int chunk) {
-  return row * kBgK + 16 * (chunk ^ ((row >> 2) & 3));
+  return row * kBgK + 16 * (chunk ^ ((row >> 1) & 3));
 }
 
 typedef __attribute__((address_space(3))) void lds_void_t;
@@ -857,9 +860,9 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 6 * rows * ldk);
   const __amdgpu_buffer_rsrc_t rhd = buffer_of(hdig, 5 * lenb * ldk);
   // lane L of a load instruction: tile row r0 + L/4, LDS chunk L%4, which
-  // holds global chunk (L%4) ^ ((L/16)%4) (r0 is a multiple of 16)
+  // holds global chunk (L%4) ^ ((L/8)%4) (r0 is a multiple of 16)
   const int lrow = lane >> 2;
-  const int lq = (lane & 3) ^ ((lane >> 4) & 3);
+  const int lq = (lane & 3) ^ ((lane >> 3) & 3);
   // C: 6 planes x 8 instructions of 16 rows; H: 5 planes x 4; wave w takes
   // instructions w, w + 4, ...  Byte offsets at k0 = 0, once: a row past
   // the matrix gets kBdNoRow, which stays past the end of the digit planes
