@@ -805,12 +805,12 @@ k_hbb_digits(int64_t lenb, int64_t ldk, const double* __restrict__ hcol,
 }
 
 // GEMM tile: 128 M rows (64 alpha strings, real and imaginary rows) x 64
-// columns per workgroup of 4 waves (2 x 2, 64 x 32 per wave: 4 x 2 MFMA
-// tiles of 16 x 16, six accumulators each: 192 accumulator registers, so
-// one wave per SIMD with the accumulators in AGPRs), K in steps of 64.
-// Against 32 x 32 wave tiles at two waves per SIMD this halves the LDS
-// operand reads per MFMA (each A fragment serves two column tiles, each B
-// fragment four row tiles) and the barriers per MFMA.
+// columns per workgroup of 8 waves (4 x 2, 32 x 32 per wave: 2 x 2 MFMA
+// tiles of 16 x 16, six accumulators each, 96 registers), two waves per
+// SIMD, K in steps of 64.  With 64 x 32 per wave (192 accumulators, one
+// wave per SIMD) the LDS operand reads per MFMA were fewer, but every wait
+// for a K step's loads and fragments was exposed: 1.59 ms instead of 1.35
+// at norb 14.
 // The operand tiles are double-buffered in LDS (2 x 68 KB, one workgroup
 // per CU) and filled by direct buffer-to-LDS loads (buffer_load_dwordx4
 // ... lds: lane L of an instruction writes bytes 16 L .. 16 L + 15 past the
@@ -833,7 +833,7 @@ __device__ __forceinline__ int bd_off(int row, int chunk) {
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1)))
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
                   const uint8_t* __restrict__ cdig,
                   const double* __restrict__ cscale,
@@ -855,7 +855,7 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   const int64_t rows = 2 * lena;
   const int64_t m0 = static_cast<int64_t>(im) * kBgTM;
   const int64_t n0 = static_cast<int64_t>(in) * kBgTN;
-  const int wm = (wave >> 1) * 64;
+  const int wm = (wave >> 1) * 32;
   const int wn = (wave & 1) * 32;
   const __amdgpu_buffer_rsrc_t rcd = buffer_of(cdig, 6 * rows * ldk);
   const __amdgpu_buffer_rsrc_t rhd = buffer_of(hdig, 5 * lenb * ldk);
@@ -864,45 +864,48 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   const int lrow = lane >> 2;
   const int lq = (lane & 3) ^ ((lane >> 3) & 3);
   // C: 6 planes x 8 instructions of 16 rows; H: 5 planes x 4; wave w takes
-  // instructions w, w + 4, ...  Byte offsets at k0 = 0, once: a row past
+  // instructions w, w + 8, ... (six C, and three or two H: waves 0..3 take
+  // H instruction 16 + w)  Byte offsets at k0 = 0, once: a row past
   // the matrix gets kBdNoRow, which stays past the end of the digit planes
   // for every k0 (their size is below kBdNoRow), so it reads zero.
-  int coff[12], hoff[5];
+  int coff[6], hoff[3];
 #pragma unroll
-  for (int j = 0; j < 12; ++j) {
-    const int ins = wave + 4 * j;
+  for (int j = 0; j < 6; ++j) {
+    const int ins = wave + 8 * j;
     const int i = ins >> 3, r0 = (ins & 7) * 16;
     const int64_t gr = m0 + r0 + lrow;
     coff[j] = gr < rows ? static_cast<int>((i * rows + gr) * ldk + 16 * lq) : kBdNoRow;
   }
 #pragma unroll
-  for (int j = 0; j < 5; ++j) {
-    const int ins = wave + 4 * j;
+  for (int j = 0; j < 3; ++j) {
+    const int ins = wave + 8 * j;
     const int i = ins >> 2, r0 = (ins & 3) * 16;
     const int64_t gn = n0 + r0 + lrow;
-    hoff[j] = gn < lenb ? static_cast<int>((i * lenb + gn) * ldk + 16 * lq) : kBdNoRow;
+    hoff[j] = (ins < 20 && gn < lenb) ? static_cast<int>((i * lenb + gn) * ldk + 16 * lq) : kBdNoRow;
   }
   auto issue = [&](uint8_t* base, int k0) {
 #pragma unroll
-    for (int j = 0; j < 12; ++j) {
-      const int ins = wave + 4 * j;
+    for (int j = 0; j < 6; ++j) {
+      const int ins = wave + 8 * j;
       const int i = ins >> 3, r0 = (ins & 7) * 16;
       __builtin_amdgcn_raw_ptr_buffer_load_lds(
           rcd, (lds_void_t*)(base + (i * kBgTM + r0) * kBgK), 16, coff[j] + k0,
           0, 0, 0);
     }
 #pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      const int ins = wave + 4 * j;
-      const int i = ins >> 2, r0 = (ins & 3) * 16;
-      __builtin_amdgcn_raw_ptr_buffer_load_lds(
-          rhd, (lds_void_t*)(base + 6 * kBgTM * kBgK + (i * kBgTN + r0) * kBgK),
-          16, hoff[j] + k0, 0, 0, 0);
+    for (int j = 0; j < 3; ++j) {
+      const int ins = wave + 8 * j;
+      if (ins < 20) {
+        const int i = ins >> 2, r0 = (ins & 3) * 16;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            rhd, (lds_void_t*)(base + 6 * kBgTM * kBgK + (i * kBgTN + r0) * kBgK),
+            16, hoff[j] + k0, 0, 0, 0);
+      }
     }
   };
-  int4v acc[4][2][6];
+  int4v acc[2][2][6];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -921,7 +924,7 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
         b[y][i] = *reinterpret_cast<const int4v*>(
             sh + i * kBgTN * kBgK + bd_off(wn + 16 * y + (lane & 15), q));
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
+    for (int x = 0; x < 2; ++x) {
       int4v a[6];
 #pragma unroll
       for (int i = 0; i < 6; ++i)
@@ -952,7 +955,7 @@ k_beta_gemm_oz(int64_t lena, int64_t lenb, int64_t ldk,
   // output item e of lane l: M row 4 (l/16) + e, column l%16; rows 2 Ia
   // (real) and 2 Ia + 1 (imaginary) are items e and e + 1
 #pragma unroll
-  for (int x = 0; x < 4; ++x)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y) {
       const int64_t col = n0 + wn + 16 * y + (lane & 15);
@@ -1016,7 +1019,7 @@ int launch_beta_gemm_oz(int64_t lena, int64_t lenb, const void* hoz,
   const int tm = static_cast<int>((2 * lena + kBgTM - 1) / kBgTM);
   const int tn = static_cast<int>((lenb + kBgTN - 1) / kBgTN);
   const int per_xcd = (tm * tn + kBgXcds - 1) / kBgXcds;
-  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 256, 0, st>>>(
+  k_beta_gemm_oz<<<static_cast<unsigned>(per_xcd * kBgXcds), 512, 0, st>>>(
       lena, lenb, ldk, cdig, cscale, hdig, hscale, hcorr, out, done, tm, tn,
       per_xcd);
   FQEHIP_CHECK_LAUNCH("k_beta_gemm_oz");
