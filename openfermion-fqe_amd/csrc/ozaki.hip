@@ -457,38 +457,19 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
       const uint32_t ib = static_cast<uint32_t>(colok ? jb : 0);
       int4v dd[2][kOzDDig][kOzKSteps];      // [re, im][digit][k step]
       // batch b: k = 64 (b / 4) + 16 kq + 4 (b % 4) + u, u = 0..3
-      u32x4 la[2][4], lb[2][4];
-      auto issue = [&](int b) {
-        const int s = b >> 2, v = b & 3;
-        const int q0 = 64 * s + 16 * kq + 4 * v;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          la[b & 1][u] = __builtin_amdgcn_raw_buffer_load_b128(
-              rc, static_cast<int>((goff[q0 + u] + ib) << 4), 0, 0);
-          const uint32_t o = tw[s][v][u] & 0x7fffffffu;
-          if (ROW_LDS)
-            lb[b & 1][u] = *reinterpret_cast<const u32x4*>(crow + o);
-          else
-            lb[b & 1][u] = __builtin_amdgcn_raw_buffer_load_b128(rrow, static_cast<int>(o), 0, 0);
-        }
-      };
-      issue(0);
-#pragma unroll
-      for (int b = 0; b < 4 * kOzKSteps; ++b) {
-        if (b + 1 < 4 * kOzKSteps) issue(b + 1);
+      // batch b: digits of D' for k = 64 (b / 4) + 16 kq + 4 (b % 4) + u
+      auto digits = [&](int b, const u32x4 (&A)[4], const u32x4 (&B)[4]) {
         const int s = b >> 2, v = b & 3;
         const int q0 = 64 * s + 16 * kq + 4 * v;
         uint32_t lre[4], hre[4], lim[4], him[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          const u32x4 A = la[b & 1][u];
-          const u32x4 B = lb[b & 1][u];
           const uint32_t sg = tw[s][v][u] & 0x80000000u;
           const double sa = gsa[q0 + u];
-          const double vre = fma(sa, as_double(A[0], A[1]),
-                                 fma(dsc2, as_double(B[0], B[1] ^ sg), kMagic));
-          const double vim = fma(sa, as_double(A[2], A[3]),
-                                 fma(dsc2, as_double(B[2], B[3] ^ sg), kMagic));
+          const double vre = fma(sa, as_double(A[u][0], A[u][1]),
+                                 fma(dsc2, as_double(B[u][0], B[u][1] ^ sg), kMagic));
+          const double vim = fma(sa, as_double(A[u][2], A[u][3]),
+                                 fma(dsc2, as_double(B[u][2], B[u][3] ^ sg), kMagic));
           split_words(vre, lre[u], hre[u]);
           split_words(vim, lim[u], him[u]);
         }
@@ -505,6 +486,46 @@ k_split_rows_oz(int norb, int compressed, int npair, int64_t lena,
           dd[1][i][s][v] = static_cast<int>(oim[i]);
         }
         asm volatile("" ::: "memory");
+      };
+      auto alpha_load = [&](int b, u32x4 (&A)[4]) {
+        const int q0 = 64 * (b >> 2) + 16 * kq + 4 * (b & 3);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          A[u] = __builtin_amdgcn_raw_buffer_load_b128(
+              rc, static_cast<int>((goff[q0 + u] + ib) << 4), 0, 0);
+      };
+      if constexpr (ROW_LDS) {
+        // global alpha gathers two batches ahead; the beta gathers from
+        // the LDS row at use
+        u32x4 la[3][4];
+        alpha_load(0, la[0]);
+        alpha_load(1, la[1]);
+#pragma unroll
+        for (int b = 0; b < 4 * kOzKSteps; ++b) {
+          if (b + 2 < 4 * kOzKSteps) alpha_load(b + 2, la[(b + 2) % 3]);
+          const int s = b >> 2, v = b & 3;
+          u32x4 lb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            lb[u] = *reinterpret_cast<const u32x4*>(crow + (tw[s][v][u] & 0x7fffffffu));
+          digits(b, la[b % 3], lb);
+        }
+      } else {
+        u32x4 la[2][4], lb[2][4];
+        auto issue = [&](int b) {
+          const int s = b >> 2, v = b & 3;
+          alpha_load(b, la[b & 1]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            lb[b & 1][u] = __builtin_amdgcn_raw_buffer_load_b128(
+                rrow, static_cast<int>(tw[s][v][u] & 0x7fffffffu), 0, 0);
+        };
+        issue(0);
+#pragma unroll
+        for (int b = 0; b < 4 * kOzKSteps; ++b) {
+          if (b + 1 < 4 * kOzKSteps) issue(b + 1);
+          digits(b, la[b & 1], lb[b & 1]);
+        }
       }
       // the next chunk's table (the next item's first chunk after the last)
       load_table(chunk + 1 < c1 ? chunk + 1 : next_c0);
