@@ -209,9 +209,13 @@ typedef struct fqehip_sector {
  * products of 8-bit digits on the int8 matrix cores (Ozaki scheme: A' in
  * five balanced digits per pair row, D' as 48-bit integers per alpha
  * string, 20 digit pairs, int32 accumulation, FP64 assembly); complex128
- * storage and FP64 slot rows as on the FP64 path.  The workspace grows by
- * lena doubles (row maxima of |C|) and, with hbb_oz set, by the C digit
- * planes of the beta-beta GEMM (6 * 2 lena * ldk bytes + lena doubles). */
+ * storage and row accumulators, the slot rows in 48 bits per double (12
+ * bytes per complex value, relative rounding at most 2^-37) within the
+ * FP64 path's slot region, which fqehip_apply_slot_reduce reads back for
+ * the same sector.  The workspace grows by the beta gather table and the
+ * pair rows' digits, lena doubles (row maxima of |C|) and, with hbb_oz
+ * set, by the C digit planes of the beta-beta GEMM (6 * 2 lena * ldk bytes
+ * + lena doubles). */
 #define FQEHIP_TILE_OZAKI 64
 
 typedef struct fqehip_operator {
@@ -237,7 +241,9 @@ int fqehip_fused_tables(int norb, int nele, int64_t nstrings,
  * fused tables set in the sector, fqehip_fused_supported() true.
  *   fused_rows: slot rows and row accumulators into d_workspace;
  *   slot_reduce: out = alpha * (H C) + beta * cin with the epilogue of
- *   fqehip_scatter. */
+ *   fqehip_scatter.
+ * The slot rows' format follows the sector (FQEHIP_TILE_OZAKI with
+ * hbb_f64: 48-bit), so both calls take the same sector. */
 int fqehip_fused_supported(int precision, const fqehip_sector* s,
                            const fqehip_operator* op);
 
