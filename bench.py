@@ -460,10 +460,12 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
     hi/lo copies of Hbb for the mixed engine, FP64 columns for FP64).
     The Ozaki-digit engine moves the FP64 engine's bytes except for the
     slot rows, which it writes and the reduction reads in 48 bits per
-    double (12 bytes per complex value); its executed int8 operations
-    ("ops") are 20 digit products of the padded tiles: 64 live rows x 128
-    gathered rows per column (alpha rows) and 2 lena x ldk x lenb
-    (beta-beta GEMM, ldk = lenb rounded up to 64)."""
+    double (12 bytes per complex value).  Its algorithmic int8 operations
+    ("ops") are the scheme's 20 digit products per product of the FP64
+    contraction: over the nslot + 1 live pair rows and npair + 1 gathered
+    rows per column (alpha rows) and 2 lena x lenb x lenb (beta-beta GEMM);
+    "executed_ops" are those of the padded tiles the kernels run (64 rows x
+    128 gathered rows; K = lenb rounded up to 64)."""
     from fqe_amd import settings
     p = op.npair
     nslot = graph.nalpha() * (graph.norb() - graph.nalpha())
@@ -478,9 +480,11 @@ def kernel_work(op, graph, n, esz, mixed, ozaki=False):
         lena = graph.lena()
         return {
             "beta_gemm": {"bytes": 2 * 2 * n * esz + 5 * lenb * ldk,
-                          "ops": 20 * 2.0 * (2 * lena) * ldk * lenb},
+                          "ops": 20 * 2.0 * (2 * lena) * lenb * lenb,
+                          "executed_ops": 20 * 2.0 * (2 * lena) * ldk * lenb},
             "alpha_rows": {"bytes": nslot * 12 * n + 3 * 2 * n * esz,
-                           "ops": 20 * 2.0 * 64 * 128 * 2 * n},
+                           "ops": 20 * 2.0 * (nslot + 1) * (p + 1) * 2 * n,
+                           "executed_ops": 20 * 2.0 * 64 * 128 * 2 * n},
             "slot_reduce": {"bytes": nslot * 12 * n + 2 * 2 * n * esz},
         }
     return {
@@ -533,8 +537,9 @@ def roofline_entry(name, per_ms, work, mode, traffic=None):
     t = per_ms[name] * 1e-3
     w = work[name]
     if mode == "ozaki":
-        # int8 digit products (executed operations) against the int8 MFMA
-        # peak, or the FP64 engine's algorithmic bytes against HBM
+        # the scheme's int8 digit products (algorithmic operations, no tile
+        # padding) against the int8 MFMA peak, or its algorithmic bytes
+        # against HBM
         t_hbm = w["bytes"] / (HBM_PEAK_GBS * 1e9)
         t_mfma = w.get("ops", 0.0) / (PEAK_TFLOPS["i8"] * 1e12)
         if t_mfma > t_hbm:
@@ -543,6 +548,7 @@ def roofline_entry(name, per_ms, work, mode, traffic=None):
                     "peak": PEAK_TFLOPS["i8"], "unit": "TOP/s (int8)",
                     "frac": round(ach / PEAK_TFLOPS["i8"], 4),
                     "traffic": traffic, "ops_per_launch": w["ops"],
+                    "executed_ops_per_launch": w.get("executed_ops"),
                     "algorithmic_bytes_per_launch": w["bytes"]}
         ach = w["bytes"] / t / 1e9
         return {"kernel": name, "bound": "hbm", "achieved": round(ach, 1),
